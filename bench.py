@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark: the batched multi-bounce sweep on one MI355X.
+
+Workload (one step): trihedral corner reflector + open duct (34,816
+triangles), f = 10 GHz, lambda/10 ray spacing (1.6 m launch grid, 535 x 535
+rays), max 10 bounces, theta 0..90 x phi 0..90 in 1 degree steps (8,281
+angles), all angles in one persistent launch, FP32 traversal with FP64 phase
+and a compensated FP32 complex sum. Mesh, BVH and the job list are resident
+in HBM before the timed region.
+
+Prints one JSON line (metric: angles/s; also ray-bounces/s, the per-angle
+FP32 method on the same GPU, accuracy vs FP64, roofline and a CPU baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 32  # 6 floats of bounds + 2 ints, per node visit of the reference traversal
+TRI_BYTES = 36   # 9 floats, per Moller-Trumbore test
+OUT_BYTES = 16   # one complex per angle
+
+WORKLOAD = dict(theta_start=0.0, theta_end=90.0, theta_samples=91, phi_start=0.0, phi_end=90.0, phi_samples=91,
+                grid_size=1.6, n_x=535, n_y=535, max_bounces=10, freq=10e9, reflection_const=1.0)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="mixed", choices=["fp32", "fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the baseline sample")
+    ap.add_argument("--per-angle-steps", type=int, default=1)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (rocprof FETCH/WRITE_SIZE calibration)")
+    return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+
+    n = 1 << 28  # 2^28 floats = 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+
+
+def cpu_baseline(verts, faces, kw, target_s):
+    """The oracle (a C++/OpenMP restatement of the reference path; the
+    reference has no CPU ray path) timed on the host cores over a bounded
+    sample of the same sweep's angles."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    p = O.Params(**{k: v for k, v in kw.items()})
+    sc = O.Scene(verts, faces, precision=0)
+    n = p.n_angles
+    # evenly spread probe angles; grow the sample until ~target_s
+    order = np.random.default_rng(7).permutation(n).astype(np.int32)
+    t0 = time.perf_counter()
+    sc.sweep(p, order[:2])
+    per = (time.perf_counter() - t0) / 2
+    m = int(max(2, min(n, target_s / max(per, 1e-6))))
+    sample = order[:m]
+    t0 = time.perf_counter()
+    sc.sweep(p, sample)
+    dt = time.perf_counter() - t0
+    return {"value": m / dt, "unit": "angles/s", "cores": O.lib().orc_num_threads(), "kind": "port",
+            "sample": f"{m} of {n} angles (seeded random), FP32 oracle (reference arithmetic), {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sweep would shard by angle "
+              "tiles across GPUs, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    import torch  # device plumbing only: the sweep runs in libsbr.so
+
+    import sagittasbr_amd as S
+    from sagittasbr_amd import meshes as M
+
+    prec = {"fp32": S.SBR_FP32, "fp64": S.SBR_FP64, "mixed": S.SBR_MIXED}[args.precision]
+    verts, faces = M.trihedral_duct_scene()
+    params = S.SweepParams.make(**WORKLOAD)
+    ctx = S.Context(0)
+    mesh = ctx.mesh(verts, faces)
+    jobs = S.sweep_jobs(params, mesh, prec)
+    plan = ctx.plan(jobs, prec, S.SBR_BATCHED)
+    n_angles = params.n_angles
+
+    for _ in range(args.warmup):
+        plan.run()
+        plan.wait()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run()
+        plan.wait()
+        kernel_ms.append(plan.last_ms())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    z, sigma, dbsm = plan.results()
+
+    # work counters (separate, untimed run) -> ray-bounces and algorithmic bytes
+    cnt = plan.count()
+    alg_bytes = cnt["node_visits"] * NODE_BYTES + cnt["tri_tests"] * TRI_BYTES + OUT_BYTES * n_angles
+    k_ms = float(np.mean(kernel_ms))
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+
+    # accuracy vs the FP64 path on the same GPU
+    jobs64 = S.sweep_jobs(params, mesh, S.SBR_FP64)
+    p64 = ctx.plan(jobs64, S.SBR_FP64, S.SBR_BATCHED)
+    p64.run()
+    p64.wait()
+    z64, s64, db64 = p64.results()
+    fp64_ms = p64.last_ms()
+    derr = np.abs(dbsm - db64)
+    peak_db = db64.max()
+    strong = db64 > peak_db - 30.0
+
+    # plain per-angle FP32 kernel on the same GPU (one launch per angle)
+    jobs32 = S.sweep_jobs(params, mesh, S.SBR_FP32)
+    pa = ctx.plan(jobs32, S.SBR_FP32, S.SBR_PER_ANGLE)
+    pa.run()
+    pa.wait()
+    pa_t = []
+    for _ in range(max(1, args.per_angle_steps)):
+        t1 = time.perf_counter()
+        pa.run()
+        pa.wait()
+        pa_t.append(time.perf_counter() - t1)
+    per_angle_rate = n_angles / float(np.mean(pa_t))
+
+    value = n_angles / (ms_per_step / 1000.0)
+    out = {
+        "metric": "angles/s (batched mixed-precision sweep, <=0.1 dB vs FP64), 1 GPU",
+        "value": round(value, 2),
+        "unit": "angles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"mixed": "f32+f64", "fp32": "f32", "fp64": "f64"}[args.precision],
+        "data": "synthetic (procedural trihedral + duct mesh)",
+        "config": {"workload": "trihedral+duct multi-bounce sweep: 34,816 triangles, 10 GHz, 535x535 rays "
+                               "(lambda/10 over 1.6 m), max 10 bounces, theta 0..90 x phi 0..90 (8,281 angles), "
+                               "one persistent launch", "method": "batched", "precision": args.precision},
+        "ray_bounces_per_s": round(cnt["ray_bounces"] / (ms_per_step / 1000.0), 1),
+        "rays_traced_per_s": round(cnt["rays"] / (ms_per_step / 1000.0), 1),
+        "per_angle_fp32_angles_per_s": round(per_angle_rate, 2),
+        "batched_over_per_angle": round(value / per_angle_rate, 3),
+        "fp64_angles_per_s": round(n_angles / (fp64_ms / 1000.0), 2),
+        "accuracy_vs_fp64_db": {"max": float(derr.max()), "p99": float(np.percentile(derr, 99)),
+                                "median": float(np.median(derr)),
+                                "max_within_30dB_of_peak": float(derr[strong].max()),
+                                "frac_angles_le_0.1dB": float(np.mean(derr <= 0.1))},
+        "counters": cnt,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel_ms": round(k_ms, 3), "algorithmic_bytes": int(alg_bytes)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(verts, faces, WORKLOAD, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,185 @@
+/*
+ * sbr.h — C ABI of the MI355X shooting-and-bouncing-rays engine (libsbr.so).
+ *
+ * The reference (marco-pas/SagittaSBR) has no plugin or FFI layer; its seams
+ * are internal C++ functions. Each entry point below names the reference
+ * interface it replaces (paths relative to the reference repository).
+ *
+ * Conventions
+ *   - Every call returns an int status: 0 = success, nonzero = failure; the
+ *     message is then available from sbr_last_error(ctx) (or
+ *     sbr_last_error(NULL) for failures without a context). The library never
+ *     calls exit() (the reference's checkCudaErrors does,
+ *     src/cuda/cudaUtils.cu:6-14).
+ *   - A context is bound to one device and one HIP stream. Calls on one
+ *     context are not thread-safe; separate contexts are independent.
+ *   - The caller owns all host arrays passed in or out. The context owns all
+ *     device memory; nothing is allocated on the device inside a sweep once
+ *     a plan exists.
+ *   - Precision is a runtime choice (the reference's compile-time USE_DOUBLE
+ *     switch, include/RT/precision.h:29-101):
+ *       SBR_FP32   FP32 traversal and physics (the reference's float build)
+ *       SBR_FP64   FP64 throughout (the reference's USE_DOUBLE build)
+ *       SBR_MIXED  FP32 traversal, FP64 path length and phase, FP32
+ *                  two-sum compensated complex sum
+ */
+#ifndef SBR_H
+#define SBR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBR_ABI_VERSION 1
+
+typedef struct sbr_ctx sbr_ctx;
+typedef struct sbr_plan sbr_plan;
+
+enum sbr_precision { SBR_FP32 = 0, SBR_FP64 = 1, SBR_MIXED = 2 };
+
+enum sbr_method {
+    SBR_PER_ANGLE = 0, /* one launch per angle, the reference's pattern (src/app/sweep.cu:332-435) */
+    SBR_BATCHED = 1    /* one persistent launch draining a device tile queue */
+};
+
+/* BVH build options; mirrors bvhBuildOptions (include/scene/bvhBuilder.hpp:13-20). */
+typedef struct {
+    int leaf_size;                  /* default 4  */
+    int bin_count;                  /* default 16 */
+    int max_parallel_depth;         /* default 6  */
+    int min_triangles_for_parallel; /* default 4096 */
+    int enable_parallel;            /* default 1  */
+    int algorithm;                  /* 0 = median split only, 1 = binned SAH (default) */
+} sbr_bvh_options;
+
+/* Angular sweep description; mirrors simulationConfig (include/app/config.hpp:7-27). */
+typedef struct {
+    double phi_start, phi_end;
+    int phi_samples;
+    double theta_start, theta_end;
+    int theta_samples;
+    double grid_size;
+    int n_x, n_y;
+    int max_bounces;
+    double reflection_const;
+    double freq;
+} sbr_sweep_params;
+
+/* One sigma value: (mesh, frequency, theta, phi) with its launch grid. */
+typedef struct {
+    int mesh;
+    int max_bounces;
+    int n_x, n_y;
+    double freq;
+    double theta_deg, phi_deg; /* converted to the working precision before use */
+    double grid_size;
+    double reflection_const;
+} sbr_job;
+
+/* Launch-shape knobs chosen by the autotuner (0 = library default). */
+typedef struct {
+    int wg_per_cu;   /* persistent workgroups per CU */
+    int lds_stack;   /* traversal stack entries kept in LDS per ray */
+    int tile_rays;   /* rays per tile side (tiles are square, multiple of 8) */
+    int reserved;
+} sbr_tuning;
+
+/* Mesh / BVH statistics. */
+typedef struct {
+    int n_triangles;
+    int n_nodes;     /* reference-layout nodes (2N-1 bound) */
+    int n_gpu_nodes; /* internal nodes in the GPU child-box layout */
+    int depth;
+    double bounds[6];
+    double build_ms;
+} sbr_mesh_info;
+
+/* Work counters of the last plan run with counting enabled. */
+typedef struct {
+    uint64_t rays;        /* rays traced (after culling) */
+    uint64_t node_visits; /* node pops of the reference traversal order */
+    uint64_t tri_tests;   /* Moller-Trumbore tests */
+    uint64_t ray_bounces; /* sum over rays of hitCount */
+    uint64_t queries;     /* closest-hit queries */
+} sbr_counters;
+
+const char* sbr_last_error(const sbr_ctx* ctx);
+int sbr_abi_version(void);
+
+/* Replaces the device setup and buffer allocation of runRcsApp
+ * (src/app/rtRcsEntry.cu:174-196, src/app/deviceBuffers.cu:12-29). */
+int sbr_ctx_create(int device, sbr_ctx** out);
+int sbr_ctx_destroy(sbr_ctx* ctx);
+/* Number of compute units of the context's device. */
+int sbr_ctx_cu_count(const sbr_ctx* ctx, int* out);
+
+/* Triangle soup assembly (src/app/rtRcsEntry.cu:118-148), host BVH build
+ * (buildBvh, src/scene/bvhBuilder.cpp:315-360) in FP32 and FP64, and upload
+ * (src/app/rtRcsEntry.cu:181-196). xyz: nVerts*3 floats; idx: nTris*3.
+ * Out-of-range indices fail as in rtRcsEntry.cu:126-130. opts may be NULL. */
+int sbr_mesh_create(sbr_ctx* ctx, const float* xyz, int n_verts, const uint32_t* idx, int n_tris, double scale,
+                    const sbr_bvh_options* opts, int* mesh_id);
+int sbr_mesh_info_get(sbr_ctx* ctx, int mesh_id, int precision, sbr_mesh_info* out);
+/* BVH-order -> original face index permutation (n_triangles ints). */
+int sbr_mesh_permutation(sbr_ctx* ctx, int mesh_id, int precision, int32_t* perm);
+/* Reference-layout BVH export for parity checks: bounds 6*n_nodes doubles,
+ * links 4*n_nodes ints (left, right, firstTri, triCount), *root. */
+int sbr_mesh_export_bvh(sbr_ctx* ctx, int mesh_id, int precision, double* bounds6, int32_t* links4, int* root);
+
+/* Host-only BVH build, the C form of buildBvh
+ * (include/scene/bvhBuilder.hpp:28-29, src/scene/bvhBuilder.cpp:315-360)
+ * applied to the triangle soup of rtRcsEntry.cu:118-148. Needs no device.
+ * precision: SBR_FP32 builds in float, SBR_FP64 in double. */
+typedef struct sbr_bvh sbr_bvh;
+int sbr_bvh_build(const float* xyz, int n_verts, const uint32_t* idx, int n_tris, double scale,
+                  const sbr_bvh_options* opts, int precision, sbr_bvh** out);
+/* out4 = n_triangles, n_nodes, root, depth */
+int sbr_bvh_info(const sbr_bvh* b, int32_t* out4);
+/* tris9: BVH-order vertices (9 doubles per triangle); perm: BVH position ->
+ * original face; bounds6 / links4 as in sbr_mesh_export_bvh. Any may be NULL. */
+int sbr_bvh_export(const sbr_bvh* b, double* tris9, int32_t* perm, double* bounds6, int32_t* links4);
+int sbr_bvh_destroy(sbr_bvh* b);
+
+/* Angle -> launch frame of runSweep (src/app/sweep.cu:336-364) in the given
+ * precision: out12 = llc, uVec, vVec, rayDir; out_deg2 = theta, phi [deg]. */
+int sbr_angle_frame(const sbr_sweep_params* p, int angle, int precision, double* out12, double* out_deg2);
+
+/* Expand a theta/phi sweep into jobs in the reference's order
+ * (globalIdx = t*phi_samples + p, src/app/sweep.cu:332-347). */
+int sbr_sweep_jobs(const sbr_sweep_params* p, int mesh_id, int precision, sbr_job* jobs, int n_jobs);
+
+/* Plan: device-resident jobs, tiles and reduction buffers for a list of jobs.
+ * Replaces runSweep (include/sim/sweep.hpp:15-16) minus its MPI gather. */
+int sbr_plan_create(sbr_ctx* ctx, const sbr_job* jobs, int n_jobs, int precision, int method, sbr_plan** out);
+int sbr_plan_set_tuning(sbr_plan* plan, const sbr_tuning* t);
+int sbr_plan_run(sbr_plan* plan);  /* asynchronous on the context stream */
+int sbr_plan_wait(sbr_plan* plan);
+/* Device time of the last run's kernels in ms (hipEvents on the plan's stream). */
+int sbr_plan_last_ms(sbr_plan* plan, double* ms);
+/* Per-job complex sum (2 doubles per job), sigma = 4*pi*|sum|^2 and dBsm
+ * (src/app/sweep.cu:228-231); any output may be NULL. */
+int sbr_plan_results(sbr_plan* plan, double* reim, double* sigma, double* dbsm);
+/* Run once with work counters enabled (slower; not for timing). */
+int sbr_plan_count(sbr_plan* plan, sbr_counters* out);
+int sbr_plan_destroy(sbr_plan* plan);
+
+/* One-call sweep: plan_create + run + wait + results + destroy. */
+int sbr_sweep(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precision, int method, double* reim,
+              double* sigma, double* dbsm);
+
+/* Per-ray debug trace of one angle of a sweep (the reference's
+ * launchRaysMultiBounce outputs, src/cuda/rtKernels.cu:105-158, plus hit
+ * triangle ids which the reference does not record). Rays are row-major
+ * (idx = j*n_x + i). Outputs (any may be NULL): hit_ids[n_rays*max_bounces]
+ * original face index per bounce (-1 after the last hit); bounces[n_rays];
+ * path_len[n_rays]; reim_ray[2*n_rays] PO term per ray. */
+int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int angle, int precision, int32_t* hit_ids,
+                   int32_t* bounces, double* path_len, double* reim_ray);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBR_H */

@@ -1,0 +1,834 @@
+// C ABI of libsbr (declared in include/sbr.h): context, meshes, plans.
+//
+// Host-side arithmetic (launch frames, physics constants) restates
+// src/app/sweep.cu:50-55,336-364 in the working precision and is compiled
+// with -ffp-contract=off, so the frames are bit-identical to a literal
+// reading of the reference source.
+#include "sbr.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bvh_build.hpp"
+#include "sbr_device.hpp"
+
+namespace sbr {
+template <typename R>
+hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, int grid,
+                        hipStream_t s);
+hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s);
+size_t partial_bytes(bool fp64, int mode);
+template <typename R> int trace_occupancy(int mode, int sd);
+}  // namespace sbr
+
+using namespace sbr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(std::string* slot, const std::string& msg) {
+    if (slot) *slot = msg;
+    g_err = msg;
+    return 1;
+}
+
+#define HIP_TRY(slot, expr)                                                                       \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(slot, std::string(#expr) + " failed: " + hipGetErrorString(e_));          \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 256));
+        if (e == hipSuccess) bytes = std::max<size_t>(n, 256);
+        return e;
+    }
+};
+
+template <typename R> struct PrecMesh {
+    bool built = false;
+    Bvh<R> bvh;
+    GpuBvh<R> gpu;
+    DevBuf nodes, tris;
+    double buildMs = 0;
+};
+
+struct MeshEntry {
+    std::vector<float> xyz;
+    std::vector<uint32_t> idx;
+    double scale = 1.0;
+    BuildOptions opt;
+    PrecMesh<float> f32;
+    PrecMesh<double> f64;
+};
+
+}  // namespace
+
+struct sbr_ctx {
+    int device = 0;
+    int cuCount = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<std::unique_ptr<MeshEntry>> meshes;
+    DevBuf meshTable32, meshTable64;
+    bool tableDirty32 = true, tableDirty64 = true;
+};
+
+struct sbr_plan {
+    sbr_ctx* ctx = nullptr;
+    int precision = 0;
+    int method = 0;
+    int nJobs = 0;
+    std::vector<sbr_job> jobs;
+    std::vector<int64_t> tileBase;
+    std::vector<int> nTiles;
+    int64_t totalTiles = 0;
+    int maxDepth = 0;
+    sbr_tuning tune{0, 0, 0, 0};
+    int tileSide = 16;
+    DevBuf dJobs, dPartials, dJobOut, dJobDone, dCounter, dCounters;
+    std::vector<double> out;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ran = false;
+};
+
+namespace {
+
+template <typename R> PrecMesh<R>& prec_of(MeshEntry& m);
+template <> PrecMesh<float>& prec_of<float>(MeshEntry& m) { return m.f32; }
+template <> PrecMesh<double>& prec_of<double>(MeshEntry& m) { return m.f64; }
+
+// Triangle soup in R (src/app/rtRcsEntry.cu:118-148): float position
+// converted to R, times the model scale in R.
+template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
+    PrecMesh<R>& pm = prec_of<R>(m);
+    if (pm.built) return 0;
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t nt = m.idx.size() / 3;
+    std::vector<R> soup(nt * 9);
+    const R s = static_cast<R>(m.scale);
+    for (size_t f = 0; f < nt; ++f)
+        for (int c = 0; c < 3; ++c) {
+            uint32_t vi = m.idx[3 * f + c];
+            for (int k = 0; k < 3; ++k) soup[9 * f + 3 * c + k] = static_cast<R>(m.xyz[3 * size_t(vi) + k]) * s;
+        }
+    pm.bvh = build_bvh<R>(soup, m.opt);
+    if (pm.bvh.depth > kMaxDepth)
+        return fail(&ctx->err, "BVH depth " + std::to_string(pm.bvh.depth) +
+                                   " exceeds the 64-entry traversal stack of the reference traversal");
+    std::string err;
+    if (!to_gpu_layout<R>(pm.bvh, pm.gpu, err)) return fail(&ctx->err, err);
+    HIP_TRY(&ctx->err, pm.nodes.ensure(pm.gpu.nodes.size()));
+    HIP_TRY(&ctx->err, pm.tris.ensure(pm.gpu.tris.size()));
+    HIP_TRY(&ctx->err, hipMemcpy(pm.nodes.p, pm.gpu.nodes.data(), pm.gpu.nodes.size(), hipMemcpyHostToDevice));
+    HIP_TRY(&ctx->err, hipMemcpy(pm.tris.p, pm.gpu.tris.data(), pm.gpu.tris.size(), hipMemcpyHostToDevice));
+    pm.built = true;
+    pm.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (sizeof(R) == 4) ctx->tableDirty32 = true;
+    else ctx->tableDirty64 = true;
+    return 0;
+}
+
+template <typename R> int upload_table(sbr_ctx* ctx) {
+    bool& dirty = sizeof(R) == 4 ? ctx->tableDirty32 : ctx->tableDirty64;
+    DevBuf& buf = sizeof(R) == 4 ? ctx->meshTable32 : ctx->meshTable64;
+    if (!dirty && buf.p) return 0;
+    std::vector<Mesh<R>> tab(ctx->meshes.size());
+    for (size_t i = 0; i < ctx->meshes.size(); ++i) {
+        PrecMesh<R>& pm = prec_of<R>(*ctx->meshes[i]);
+        Mesh<R> mm{};
+        if (pm.built) {
+            mm.nodes = pm.nodes.p;
+            mm.tris = pm.tris.p;
+            for (int k = 0; k < 3; ++k) {
+                mm.rootLo[k] = pm.gpu.rootLo[k];
+                mm.rootHi[k] = pm.gpu.rootHi[k];
+            }
+            mm.rootRef = pm.gpu.rootRef;
+            mm.nTris = static_cast<int>(pm.bvh.perm.size());
+        }
+        tab[i] = mm;
+    }
+    HIP_TRY(&ctx->err, buf.ensure(sizeof(Mesh<R>) * std::max<size_t>(tab.size(), 1)));
+    HIP_TRY(&ctx->err, hipMemcpy(buf.p, tab.data(), sizeof(Mesh<R>) * tab.size(), hipMemcpyHostToDevice));
+    dirty = false;
+    return 0;
+}
+
+template <typename R> struct V {
+    R x, y, z;
+};
+template <typename R> V<R> crossp(V<R> a, V<R> b) {
+    return {a.y * b.z - a.z * b.y, -(a.x * b.z - a.z * b.x), a.x * b.y - a.y * b.x};
+}
+template <typename R> V<R> unitp(V<R> v) {
+    R len = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    return {v.x / len, v.y / len, v.z / len};
+}
+
+// Angle of sweep index -> degrees in R (src/app/sweep.cu:336-347).
+template <typename R> void sweep_degrees(const sbr_sweep_params* p, int idx, R& th, R& ph) {
+    const int t = idx / p->phi_samples, q = idx % p->phi_samples;
+    const R ts = R(p->theta_start), te = R(p->theta_end), ps = R(p->phi_start), pe = R(p->phi_end);
+    th = ts + (p->theta_samples > 1 ? t * (te - ts) / (p->theta_samples - 1) : R(0));
+    ph = ps + (p->phi_samples > 1 ? q * (pe - ps) / (p->phi_samples - 1) : R(0));
+}
+
+// Launch frame (src/app/sweep.cu:349-364).
+template <typename R> void make_frame(R thDeg, R phDeg, R grid, R llc[3], R uv[3], R vv[3], R dir[3]) {
+    const R pi = R(M_PI);
+    R th = thDeg * pi / R(180.0);
+    R ph = phDeg * pi / R(180.0);
+    V<R> d{std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
+    V<R> up{R(0), R(1), R(0)};
+    if (std::fabs(d.y) > R(0.99)) up = {R(1), R(0), R(0)};
+    V<R> u = unitp(crossp(up, d));
+    u = {u.x * grid, u.y * grid, u.z * grid};
+    V<R> v = unitp(crossp(d, u));
+    v = {v.x * grid, v.y * grid, v.z * grid};
+    const R off = R(10.0) * grid;
+    llc[0] = (d.x * off - R(0.5) * u.x) - R(0.5) * v.x;
+    llc[1] = (d.y * off - R(0.5) * u.y) - R(0.5) * v.y;
+    llc[2] = (d.z * off - R(0.5) * u.z) - R(0.5) * v.z;
+    uv[0] = u.x; uv[1] = u.y; uv[2] = u.z;
+    vv[0] = v.x; vv[1] = v.y; vv[2] = v.z;
+    dir[0] = -d.x; dir[1] = -d.y; dir[2] = -d.z;
+}
+
+// Physics constants (src/app/sweep.cu:50-55, poKernels.cu:98-101).
+template <typename R> void make_phys(const sbr_job& j, Job<R>& J) {
+    const R c0 = R(299792458.0);
+    R lambda = c0 / R(j.freq);
+    J.k = R(2.0) * R(M_PI) / lambda;
+    R g = R(j.grid_size);
+    R area = (g * g) / R(j.n_x * j.n_y);
+    J.twoPi = R(2.0) * R(M_PI);
+    J.pre = (J.k * area / (R(4.0) * R(M_PI))) * R(2.0);
+    J.refl = R(j.reflection_const);
+    J.kd = 2.0 * M_PI / (299792458.0 / j.freq);
+    J.twoPid = 2.0 * M_PI;
+}
+
+// Conservative ray-index rectangle whose rays can reach the root box:
+// the 8 corners of the box are projected along the ray direction onto the
+// launch plane (exact 3x3 solve in double), plus a 2-ray margin.
+template <typename R>
+bool cull_rect(const Job<R>& J, const R lo[3], const R hi[3], int& i0, int& i1, int& j0, int& j1) {
+    double U[3], W[3], D[3], C[3];
+    for (int k = 0; k < 3; ++k) {
+        U[k] = J.uv[k];
+        W[k] = J.vv[k];
+        D[k] = J.dir[k];
+        C[k] = J.llc[k];
+    }
+    // columns U, W, D; solve M [a b c]^T = X - llc by Cramer's rule
+    auto det3 = [](const double* a, const double* b, const double* c) {
+        return a[0] * (b[1] * c[2] - b[2] * c[1]) - b[0] * (a[1] * c[2] - a[2] * c[1]) +
+               c[0] * (a[1] * b[2] - a[2] * b[1]);
+    };
+    double det = det3(U, W, D);
+    if (!(std::fabs(det) > 0)) return false;
+    double amin = 1e300, amax = -1e300, bmin = 1e300, bmax = -1e300;
+    for (int c = 0; c < 8; ++c) {
+        double X[3] = {double((c & 1) ? hi[0] : lo[0]) - C[0], double((c & 2) ? hi[1] : lo[1]) - C[1],
+                       double((c & 4) ? hi[2] : lo[2]) - C[2]};
+        double a = det3(X, W, D) / det;
+        double b = det3(U, X, D) / det;
+        amin = std::min(amin, a);
+        amax = std::max(amax, a);
+        bmin = std::min(bmin, b);
+        bmax = std::max(bmax, b);
+    }
+    // ray (i, j) sits at a = (i + 0.5) / nx
+    double fi0 = std::floor(amin * J.nx - 0.5) - 2, fi1 = std::ceil(amax * J.nx - 0.5) + 2;
+    double fj0 = std::floor(bmin * J.ny - 0.5) - 2, fj1 = std::ceil(bmax * J.ny - 0.5) + 2;
+    fi0 = std::max(fi0, 0.0);
+    fj0 = std::max(fj0, 0.0);
+    fi1 = std::min(fi1, double(J.nx - 1));
+    fj1 = std::min(fj1, double(J.ny - 1));
+    if (fi0 > fi1 || fj0 > fj1) return false;
+    i0 = int(fi0); i1 = int(fi1); j0 = int(fj0); j1 = int(fj1);
+    return true;
+}
+
+template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
+    sbr_ctx* ctx = pl->ctx;
+    std::vector<Job<R>> jobs(pl->nJobs);
+    pl->tileBase.assign(pl->nJobs, 0);
+    pl->nTiles.assign(pl->nJobs, 0);
+    int64_t base = 0;
+    int maxDepth = 0;
+    const int ts = pl->tileSide;
+    for (int i = 0; i < pl->nJobs; ++i) {
+        const sbr_job& j = pl->jobs[i];
+        if (j.mesh < 0 || j.mesh >= int(ctx->meshes.size()))
+            return fail(&ctx->err, "job " + std::to_string(i) + ": bad mesh id");
+        if (j.n_x <= 0 || j.n_y <= 0 || j.max_bounces < 0 || !(j.freq > 0) || !(j.grid_size > 0))
+            return fail(&ctx->err, "job " + std::to_string(i) + ": bad grid, bounce count, frequency or size");
+        if (int64_t(j.n_x) * j.n_y > 0x7fffffffll)
+            return fail(&ctx->err, "job " + std::to_string(i) + ": n_x*n_y exceeds the int ray index");
+        MeshEntry& m = *ctx->meshes[j.mesh];
+        if (ensure_built<R>(ctx, m)) return 1;
+        PrecMesh<R>& pm = prec_of<R>(m);
+        maxDepth = std::max(maxDepth, pm.bvh.depth);
+        Job<R> J{};
+        make_frame<R>(R(j.theta_deg), R(j.phi_deg), R(j.grid_size), J.llc, J.uv, J.vv, J.dir);
+        make_phys<R>(j, J);
+        J.nx = j.n_x;
+        J.ny = j.n_y;
+        J.maxB = j.max_bounces;
+        J.mesh = j.mesh;
+        int i0 = 0, i1 = j.n_x - 1, j0 = 0, j1 = j.n_y - 1;
+        bool any = true;
+        if (!record) any = cull_rect<R>(J, pm.gpu.rootLo, pm.gpu.rootHi, i0, i1, j0, j1);
+        if (j.max_bounces == 0) any = false;
+        J.i0 = i0;
+        J.j0 = j0;
+        J.tilesX = any ? (i1 - i0 + ts) / ts : 0;
+        J.tilesY = any ? (j1 - j0 + ts) / ts : 0;
+        J.nTiles = J.tilesX * J.tilesY;
+        J.tileBase = base;
+        pl->tileBase[i] = base;
+        pl->nTiles[i] = J.nTiles;
+        base += J.nTiles;
+        jobs[i] = J;
+    }
+    pl->totalTiles = base;
+    pl->maxDepth = maxDepth;
+    if (upload_table<R>(ctx)) return 1;
+    const int mode = pl->precision == SBR_MIXED ? 1 : 0;
+    HIP_TRY(&ctx->err, pl->dJobs.ensure(sizeof(Job<R>) * std::max(pl->nJobs, 1)));
+    HIP_TRY(&ctx->err, hipMemcpy(pl->dJobs.p, jobs.data(), sizeof(Job<R>) * pl->nJobs, hipMemcpyHostToDevice));
+    HIP_TRY(&ctx->err, pl->dPartials.ensure(partial_bytes(sizeof(R) == 8, mode) * std::max<int64_t>(base, 1)));
+    HIP_TRY(&ctx->err, pl->dJobOut.ensure(sizeof(double) * 2 * std::max(pl->nJobs, 1)));
+    HIP_TRY(&ctx->err, pl->dJobDone.ensure(sizeof(unsigned) * std::max(pl->nJobs, 1)));
+    HIP_TRY(&ctx->err, pl->dCounter.ensure(sizeof(unsigned long long)));
+    HIP_TRY(&ctx->err, pl->dCounters.ensure(sizeof(unsigned long long) * 8));
+    return 0;
+}
+
+int default_sd(int depth) { return depth <= 16 ? 16 : 32; }
+
+template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int32_t* rIds, int32_t* rCnt, double* rL,
+                                      double* rReIm) {
+    sbr_ctx* ctx = pl->ctx;
+    const int mode = pl->precision == SBR_MIXED ? 1 : 0;
+    const int sd = pl->tune.lds_stack > 0 ? pl->tune.lds_stack : default_sd(pl->maxDepth);
+    TraceArgs<R> a{};
+    a.jobs = static_cast<const Job<R>*>(pl->dJobs.p);
+    a.meshes = static_cast<const Mesh<R>*>(sizeof(R) == 4 ? ctx->meshTable32.p : ctx->meshTable64.p);
+    a.nJobs = pl->nJobs;
+    a.tileSide = pl->tileSide;
+    a.totalTiles = pl->totalTiles;
+    a.partials = pl->dPartials.p;
+    a.jobOut = static_cast<double*>(pl->dJobOut.p);
+    a.jobDone = static_cast<unsigned*>(pl->dJobDone.p);
+    a.tileCounter = static_cast<unsigned long long*>(pl->dCounter.p);
+    a.recIds = rIds;
+    a.recCount = rCnt;
+    a.recL = rL;
+    a.recReIm = rReIm;
+    a.counters = static_cast<unsigned long long*>(pl->dCounters.p);
+    hipStream_t s = ctx->stream;
+    // clear job outputs, completion counters, queue head and work counters
+    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobOut.p), size_t(4) * pl->nJobs, s));
+    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobDone.p), size_t(pl->nJobs), s));
+    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounter.p), 2, s));
+    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounters.p), 16, s));
+    // the timed region covers the trace launches only
+    HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
+    if (pl->totalTiles > 0) {
+        if (pl->method == SBR_BATCHED && !record) {
+            int occ = trace_occupancy<R>(mode, sd);
+            int per = pl->tune.wg_per_cu > 0 ? pl->tune.wg_per_cu : std::max(occ, 1);
+            int64_t grid = int64_t(ctx->cuCount) * per;
+            grid = std::min<int64_t>(grid, pl->totalTiles);
+            a.tileOffset = 0;
+            HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, false, count, true, int(grid), s));
+        } else {
+            for (int i = 0; i < pl->nJobs; ++i) {
+                if (pl->nTiles[i] == 0) continue;
+                a.tileOffset = pl->tileBase[i];
+                HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, record, count, false, pl->nTiles[i], s));
+            }
+        }
+    }
+    HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
+    pl->ran = true;
+    return 0;
+}
+
+int dispatch_prepare(sbr_plan* pl, bool record) {
+    return pl->precision == SBR_FP64 ? plan_prepare<double>(pl, record) : plan_prepare<float>(pl, record);
+}
+int dispatch_launch(sbr_plan* pl, bool record, bool count, int32_t* a = nullptr, int32_t* b = nullptr,
+                    double* c = nullptr, double* d = nullptr) {
+    return pl->precision == SBR_FP64 ? plan_launch<double>(pl, record, count, a, b, c, d)
+                                     : plan_launch<float>(pl, record, count, a, b, c, d);
+}
+
+// bvhBuildOptions defaults (include/scene/bvhBuilder.hpp:13-20) for zero fields
+int parse_options(const sbr_bvh_options* o, BuildOptions& opt) {
+    if (!o) return 0;
+    opt.leafSize = o->leaf_size > 0 ? o->leaf_size : 4;
+    opt.binCount = o->bin_count > 0 ? o->bin_count : 16;
+    opt.maxParallelDepth = o->max_parallel_depth > 0 ? o->max_parallel_depth : 6;
+    opt.minTrianglesForParallel = o->min_triangles_for_parallel > 0 ? o->min_triangles_for_parallel : 4096;
+    opt.enableParallel = o->enable_parallel != 0;
+    opt.sah = o->algorithm != 0;
+    if (opt.leafSize > 16) return fail(nullptr, "leaf_size above 16 is not supported");
+    return 0;
+}
+
+bool valid_prec(int p) { return p == SBR_FP32 || p == SBR_FP64 || p == SBR_MIXED; }
+
+int create_plan(sbr_ctx* ctx, const sbr_job* jobs, int n, int precision, int method, bool record, sbr_plan** out) {
+    if (!ctx) return fail(nullptr, "null context");
+    if (!out) return fail(&ctx->err, "null plan output");
+    if (n < 0 || (n > 0 && !jobs)) return fail(&ctx->err, "bad job list");
+    if (!valid_prec(precision)) return fail(&ctx->err, "unknown precision");
+    if (method != SBR_PER_ANGLE && method != SBR_BATCHED) return fail(&ctx->err, "unknown method");
+    HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
+    auto pl = std::make_unique<sbr_plan>();
+    pl->ctx = ctx;
+    pl->precision = precision;
+    pl->method = method;
+    pl->nJobs = n;
+    pl->jobs.assign(jobs, jobs + n);
+    HIP_TRY(&ctx->err, hipEventCreate(&pl->ev0));
+    HIP_TRY(&ctx->err, hipEventCreate(&pl->ev1));
+    if (dispatch_prepare(pl.get(), record)) {
+        (void)hipEventDestroy(pl->ev0);
+        (void)hipEventDestroy(pl->ev1);
+        return 1;
+    }
+    *out = pl.release();
+    return 0;
+}
+
+template <typename R> void sigma_of(double re, double im, double* s, double* db) {
+    R r = R(re), i = R(im);
+    R sg = R(4.0) * R(M_PI) * (r * r + i * i);
+    *s = double(sg);
+    *db = double(R(10.0) * std::log10(std::fmax(sg, R(1e-10))));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sbr_last_error(const sbr_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int sbr_abi_version(void) { return SBR_ABI_VERSION; }
+
+int sbr_ctx_create(int device, sbr_ctx** out) {
+    if (!out) return fail(nullptr, "null output pointer");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(nullptr, "no HIP device available");
+    if (device < 0 || device >= n) return fail(nullptr, "device index out of range");
+    auto ctx = std::make_unique<sbr_ctx>();
+    ctx->device = device;
+    HIP_TRY(nullptr, hipSetDevice(device));
+    HIP_TRY(nullptr, hipDeviceGetAttribute(&ctx->cuCount, hipDeviceAttributeMultiprocessorCount, device));
+    HIP_TRY(nullptr, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    *out = ctx.release();
+    return 0;
+}
+
+int sbr_ctx_destroy(sbr_ctx* ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->meshes.clear();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int sbr_ctx_cu_count(const sbr_ctx* ctx, int* out) {
+    if (!ctx || !out) return fail(nullptr, "null argument");
+    *out = ctx->cuCount;
+    return 0;
+}
+
+int sbr_mesh_create(sbr_ctx* ctx, const float* xyz, int n_verts, const uint32_t* idx, int n_tris, double scale,
+                    const sbr_bvh_options* opts, int* mesh_id) {
+    if (!ctx) return fail(nullptr, "null context");
+    if (!xyz || !idx || !mesh_id || n_verts <= 0 || n_tris <= 0)
+        return fail(&ctx->err, "mesh contains no valid triangles");
+    for (int64_t i = 0; i < int64_t(n_tris) * 3; ++i)
+        if (idx[i] >= uint32_t(n_verts)) return fail(&ctx->err, "mesh contains out-of-range indices");
+    auto m = std::make_unique<MeshEntry>();
+    m->xyz.assign(xyz, xyz + size_t(n_verts) * 3);
+    m->idx.assign(idx, idx + size_t(n_tris) * 3);
+    m->scale = scale;
+    if (parse_options(opts, m->opt)) return fail(&ctx->err, g_err);
+    HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
+    ctx->meshes.push_back(std::move(m));
+    *mesh_id = int(ctx->meshes.size()) - 1;
+    // build the FP32 tree eagerly: it is the one the default paths use
+    if (ensure_built<float>(ctx, *ctx->meshes.back())) {
+        ctx->meshes.pop_back();
+        return 1;
+    }
+    return 0;
+}
+
+struct sbr_bvh {
+    int precision = 0;
+    Bvh<float> f;
+    Bvh<double> d;
+};
+
+int sbr_bvh_build(const float* xyz, int n_verts, const uint32_t* idx, int n_tris, double scale,
+                  const sbr_bvh_options* opts, int precision, sbr_bvh** out) {
+    if (!xyz || !idx || !out || n_verts <= 0 || n_tris <= 0) return fail(nullptr, "mesh contains no valid triangles");
+    if (precision != SBR_FP32 && precision != SBR_FP64) return fail(nullptr, "precision must be SBR_FP32 or SBR_FP64");
+    for (int64_t i = 0; i < int64_t(n_tris) * 3; ++i)
+        if (idx[i] >= uint32_t(n_verts)) return fail(nullptr, "mesh contains out-of-range indices");
+    BuildOptions opt;
+    if (parse_options(opts, opt)) return 1;
+    auto b = std::make_unique<sbr_bvh>();
+    b->precision = precision;
+    auto build = [&](auto zero, auto& dst) {
+        using R = decltype(zero);
+        std::vector<R> soup(size_t(n_tris) * 9);
+        const R s = static_cast<R>(scale);
+        for (size_t f = 0; f < size_t(n_tris); ++f)
+            for (int c = 0; c < 3; ++c)
+                for (int k = 0; k < 3; ++k)
+                    soup[9 * f + 3 * c + k] = static_cast<R>(xyz[3 * size_t(idx[3 * f + c]) + k]) * s;
+        dst = build_bvh<R>(soup, opt);
+    };
+    if (precision == SBR_FP64) build(0.0, b->d);
+    else build(0.0f, b->f);
+    *out = b.release();
+    return 0;
+}
+
+int sbr_bvh_info(const sbr_bvh* b, int32_t* out4) {
+    if (!b || !out4) return fail(nullptr, "null argument");
+    auto fill = [&](const auto& t) {
+        out4[0] = int32_t(t.perm.size());
+        out4[1] = int32_t(t.nodes.size());
+        out4[2] = t.root;
+        out4[3] = t.depth;
+    };
+    if (b->precision == SBR_FP64) fill(b->d);
+    else fill(b->f);
+    return 0;
+}
+
+int sbr_bvh_export(const sbr_bvh* b, double* tris9, int32_t* perm, double* bounds6, int32_t* links4) {
+    if (!b) return fail(nullptr, "null argument");
+    auto ex = [&](const auto& t) {
+        if (tris9)
+            for (size_t i = 0; i < t.tri9.size(); ++i) tris9[i] = double(t.tri9[i]);
+        if (perm) std::copy(t.perm.begin(), t.perm.end(), perm);
+        for (size_t i = 0; i < t.nodes.size(); ++i) {
+            const auto& n = t.nodes[i];
+            if (bounds6)
+                for (int k = 0; k < 3; ++k) {
+                    bounds6[6 * i + k] = n.lo[k];
+                    bounds6[6 * i + 3 + k] = n.hi[k];
+                }
+            if (links4) {
+                links4[4 * i] = n.left;
+                links4[4 * i + 1] = n.right;
+                links4[4 * i + 2] = n.firstTri;
+                links4[4 * i + 3] = n.triCount;
+            }
+        }
+    };
+    if (b->precision == SBR_FP64) ex(b->d);
+    else ex(b->f);
+    return 0;
+}
+
+int sbr_bvh_destroy(sbr_bvh* b) {
+    delete b;
+    return 0;
+}
+
+static MeshEntry* get_mesh(sbr_ctx* ctx, int id, int precision) {
+    if (id < 0 || id >= int(ctx->meshes.size())) return nullptr;
+    MeshEntry* m = ctx->meshes[id].get();
+    if (precision == SBR_FP64 ? ensure_built<double>(ctx, *m) : ensure_built<float>(ctx, *m)) return nullptr;
+    return m;
+}
+
+int sbr_mesh_info_get(sbr_ctx* ctx, int mesh_id, int precision, sbr_mesh_info* out) {
+    if (!ctx || !out) return fail(nullptr, "null argument");
+    MeshEntry* m = get_mesh(ctx, mesh_id, precision);
+    if (!m) return fail(&ctx->err, ctx->err.empty() ? "bad mesh id" : ctx->err);
+    auto fill = [&](auto& pm) {
+        out->n_triangles = int(pm.bvh.perm.size());
+        out->n_nodes = int(pm.bvh.nodes.size());
+        out->n_gpu_nodes = pm.gpu.nInternal;
+        out->depth = pm.bvh.depth;
+        for (int k = 0; k < 3; ++k) {
+            out->bounds[k] = pm.gpu.rootLo[k];
+            out->bounds[3 + k] = pm.gpu.rootHi[k];
+        }
+        out->build_ms = pm.buildMs;
+    };
+    if (precision == SBR_FP64) fill(m->f64);
+    else fill(m->f32);
+    return 0;
+}
+
+int sbr_mesh_permutation(sbr_ctx* ctx, int mesh_id, int precision, int32_t* perm) {
+    if (!ctx || !perm) return fail(nullptr, "null argument");
+    MeshEntry* m = get_mesh(ctx, mesh_id, precision);
+    if (!m) return fail(&ctx->err, "bad mesh id");
+    const std::vector<int>& p = precision == SBR_FP64 ? m->f64.bvh.perm : m->f32.bvh.perm;
+    std::copy(p.begin(), p.end(), perm);
+    return 0;
+}
+
+int sbr_mesh_export_bvh(sbr_ctx* ctx, int mesh_id, int precision, double* bounds6, int32_t* links4, int* root) {
+    if (!ctx || !bounds6 || !links4 || !root) return fail(nullptr, "null argument");
+    MeshEntry* m = get_mesh(ctx, mesh_id, precision);
+    if (!m) return fail(&ctx->err, "bad mesh id");
+    auto ex = [&](auto& pm) {
+        for (size_t i = 0; i < pm.bvh.nodes.size(); ++i) {
+            const auto& n = pm.bvh.nodes[i];
+            for (int k = 0; k < 3; ++k) {
+                bounds6[6 * i + k] = n.lo[k];
+                bounds6[6 * i + 3 + k] = n.hi[k];
+            }
+            links4[4 * i] = n.left;
+            links4[4 * i + 1] = n.right;
+            links4[4 * i + 2] = n.firstTri;
+            links4[4 * i + 3] = n.triCount;
+        }
+        *root = pm.bvh.root;
+    };
+    if (precision == SBR_FP64) ex(m->f64);
+    else ex(m->f32);
+    return 0;
+}
+
+int sbr_angle_frame(const sbr_sweep_params* p, int angle, int precision, double* out12, double* out_deg2) {
+    if (!p || !out12) return fail(nullptr, "null argument");
+    if (p->theta_samples <= 0 || p->phi_samples <= 0 || angle < 0 || angle >= p->theta_samples * p->phi_samples)
+        return fail(nullptr, "angle index out of range");
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        R th, ph, llc[3], uv[3], vv[3], dir[3];
+        sweep_degrees<R>(p, angle, th, ph);
+        make_frame<R>(th, ph, R(p->grid_size), llc, uv, vv, dir);
+        for (int k = 0; k < 3; ++k) {
+            out12[k] = llc[k];
+            out12[3 + k] = uv[k];
+            out12[6 + k] = vv[k];
+            out12[9 + k] = dir[k];
+        }
+        if (out_deg2) {
+            out_deg2[0] = th;
+            out_deg2[1] = ph;
+        }
+    };
+    if (precision == SBR_FP64) run(0.0);
+    else run(0.0f);
+    return 0;
+}
+
+int sbr_sweep_jobs(const sbr_sweep_params* p, int mesh_id, int precision, sbr_job* jobs, int n_jobs) {
+    if (!p || !jobs) return fail(nullptr, "null argument");
+    if (p->theta_samples <= 0 || p->phi_samples <= 0) return fail(nullptr, "sample counts must be positive");
+    if (n_jobs != p->theta_samples * p->phi_samples) return fail(nullptr, "n_jobs must equal theta*phi samples");
+    for (int i = 0; i < n_jobs; ++i) {
+        sbr_job& j = jobs[i];
+        j.mesh = mesh_id;
+        j.max_bounces = p->max_bounces;
+        j.n_x = p->n_x;
+        j.n_y = p->n_y;
+        j.freq = p->freq;
+        j.grid_size = p->grid_size;
+        j.reflection_const = p->reflection_const;
+        if (precision == SBR_FP64) {
+            double th, ph;
+            sweep_degrees<double>(p, i, th, ph);
+            j.theta_deg = th;
+            j.phi_deg = ph;
+        } else {
+            float th, ph;
+            sweep_degrees<float>(p, i, th, ph);
+            j.theta_deg = th;
+            j.phi_deg = ph;
+        }
+    }
+    return 0;
+}
+
+int sbr_plan_create(sbr_ctx* ctx, const sbr_job* jobs, int n_jobs, int precision, int method, sbr_plan** out) {
+    return create_plan(ctx, jobs, n_jobs, precision, method, false, out);
+}
+
+int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
+    if (!pl || !t) return fail(nullptr, "null argument");
+    if (t->lds_stack < 0 || t->wg_per_cu < 0) return fail(&pl->ctx->err, "negative tuning value");
+    if (t->tile_rays != 0 && t->tile_rays != 16 && t->tile_rays != 32 && t->tile_rays != 64)
+        return fail(&pl->ctx->err, "tile_rays must be 16, 32 or 64");
+    pl->tune = *t;
+    int ts = t->tile_rays ? t->tile_rays : 16;
+    if (ts != pl->tileSide) {
+        pl->tileSide = ts;
+        if (dispatch_prepare(pl, false)) return 1;
+    }
+    return 0;
+}
+
+int sbr_plan_run(sbr_plan* pl) {
+    if (!pl) return fail(nullptr, "null plan");
+    HIP_TRY(&pl->ctx->err, hipSetDevice(pl->ctx->device));
+    return dispatch_launch(pl, false, false);
+}
+
+int sbr_plan_wait(sbr_plan* pl) {
+    if (!pl) return fail(nullptr, "null plan");
+    HIP_TRY(&pl->ctx->err, hipStreamSynchronize(pl->ctx->stream));
+    return 0;
+}
+
+int sbr_plan_last_ms(sbr_plan* pl, double* ms) {
+    if (!pl || !ms) return fail(nullptr, "null argument");
+    if (!pl->ran) return fail(&pl->ctx->err, "plan has not run");
+    float f = 0;
+    HIP_TRY(&pl->ctx->err, hipEventSynchronize(pl->ev1));
+    HIP_TRY(&pl->ctx->err, hipEventElapsedTime(&f, pl->ev0, pl->ev1));
+    *ms = f;
+    return 0;
+}
+
+int sbr_plan_results(sbr_plan* pl, double* reim, double* sigma, double* dbsm) {
+    if (!pl) return fail(nullptr, "null plan");
+    if (!pl->ran) return fail(&pl->ctx->err, "plan has not run");
+    sbr_ctx* ctx = pl->ctx;
+    pl->out.assign(size_t(2) * pl->nJobs, 0.0);
+    HIP_TRY(&ctx->err, hipMemcpyAsync(pl->out.data(), pl->dJobOut.p, sizeof(double) * 2 * pl->nJobs,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < pl->nJobs; ++i) {
+        double re = pl->out[2 * i], im = pl->out[2 * i + 1];
+        if (reim) {
+            reim[2 * i] = re;
+            reim[2 * i + 1] = im;
+        }
+        double s = 0, d = 0;
+        if (pl->precision == SBR_FP32) sigma_of<float>(re, im, &s, &d);
+        else sigma_of<double>(re, im, &s, &d);
+        if (sigma) sigma[i] = s;
+        if (dbsm) dbsm[i] = d;
+    }
+    return 0;
+}
+
+int sbr_plan_count(sbr_plan* pl, sbr_counters* out) {
+    if (!pl || !out) return fail(nullptr, "null argument");
+    sbr_ctx* ctx = pl->ctx;
+    HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
+    if (dispatch_launch(pl, false, true)) return 1;
+    unsigned long long c[8] = {0};
+    HIP_TRY(&ctx->err, hipMemcpyAsync(c, pl->dCounters.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
+    out->rays = c[0];
+    out->node_visits = c[1];
+    out->tri_tests = c[2];
+    out->ray_bounces = c[3];
+    out->queries = c[4];
+    return 0;
+}
+
+int sbr_plan_destroy(sbr_plan* pl) {
+    if (!pl) return 0;
+    (void)hipSetDevice(pl->ctx->device);
+    (void)hipStreamSynchronize(pl->ctx->stream);
+    if (pl->ev0) (void)hipEventDestroy(pl->ev0);
+    if (pl->ev1) (void)hipEventDestroy(pl->ev1);
+    delete pl;
+    return 0;
+}
+
+int sbr_sweep(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precision, int method, double* reim,
+              double* sigma, double* dbsm) {
+    if (!ctx || !p) return fail(nullptr, "null argument");
+    const int n = p->theta_samples * p->phi_samples;
+    if (p->theta_samples <= 0 || p->phi_samples <= 0) return fail(&ctx->err, "sample counts must be positive");
+    std::vector<sbr_job> jobs(n);
+    if (sbr_sweep_jobs(p, mesh_id, precision, jobs.data(), n)) return fail(&ctx->err, g_err);
+    sbr_plan* pl = nullptr;
+    if (sbr_plan_create(ctx, jobs.data(), n, precision, method, &pl)) return 1;
+    int rc = sbr_plan_run(pl);
+    if (!rc) rc = sbr_plan_wait(pl);
+    if (!rc) rc = sbr_plan_results(pl, reim, sigma, dbsm);
+    sbr_plan_destroy(pl);
+    return rc;
+}
+
+int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int angle, int precision, int32_t* hit_ids,
+                   int32_t* bounces, double* path_len, double* reim_ray) {
+    if (!ctx || !p) return fail(nullptr, "null argument");
+    const int n = p->theta_samples * p->phi_samples;
+    if (angle < 0 || angle >= n) return fail(&ctx->err, "angle index out of range");
+    std::vector<sbr_job> all(n);
+    if (sbr_sweep_jobs(p, mesh_id, precision, all.data(), n)) return fail(&ctx->err, g_err);
+    sbr_plan* pl = nullptr;
+    if (create_plan(ctx, &all[angle], 1, precision, SBR_PER_ANGLE, true, &pl)) return 1;
+    const size_t nr = size_t(p->n_x) * p->n_y;
+    const size_t mb = std::max(p->max_bounces, 1);
+    DevBuf dIds, dCnt, dL, dRe;
+    int rc = 0;
+    auto run = [&]() -> int {
+        HIP_TRY(&ctx->err, dIds.ensure(nr * mb * sizeof(int32_t)));
+        HIP_TRY(&ctx->err, dCnt.ensure(nr * sizeof(int32_t)));
+        HIP_TRY(&ctx->err, dL.ensure(nr * sizeof(double)));
+        HIP_TRY(&ctx->err, dRe.ensure(nr * 2 * sizeof(double)));
+        HIP_TRY(&ctx->err, hipMemsetAsync(dIds.p, 0xff, nr * mb * sizeof(int32_t), ctx->stream));
+        HIP_TRY(&ctx->err, hipMemsetAsync(dCnt.p, 0, nr * sizeof(int32_t), ctx->stream));
+        HIP_TRY(&ctx->err, hipMemsetAsync(dL.p, 0, nr * sizeof(double), ctx->stream));
+        HIP_TRY(&ctx->err, hipMemsetAsync(dRe.p, 0, nr * 2 * sizeof(double), ctx->stream));
+        if (dispatch_launch(pl, true, false, static_cast<int32_t*>(dIds.p), static_cast<int32_t*>(dCnt.p),
+                            static_cast<double*>(dL.p), static_cast<double*>(dRe.p)))
+            return 1;
+        HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
+        if (hit_ids && p->max_bounces > 0)
+            HIP_TRY(&ctx->err, hipMemcpy(hit_ids, dIds.p, nr * p->max_bounces * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (bounces) HIP_TRY(&ctx->err, hipMemcpy(bounces, dCnt.p, nr * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (path_len) HIP_TRY(&ctx->err, hipMemcpy(path_len, dL.p, nr * sizeof(double), hipMemcpyDeviceToHost));
+        if (reim_ray) HIP_TRY(&ctx->err, hipMemcpy(reim_ray, dRe.p, nr * 2 * sizeof(double), hipMemcpyDeviceToHost));
+        return 0;
+    };
+    rc = run();
+    sbr_plan_destroy(pl);
+    return rc;
+}
+
+}  // extern "C"

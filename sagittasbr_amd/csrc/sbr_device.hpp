@@ -1,0 +1,57 @@
+// Plain data shared by the host driver and the HIP kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace sbr {
+
+constexpr int kWgSize = 256;   // 4 wavefronts of 64
+constexpr int kMaxStack = 64;  // deferred-left capacity; equals the reference's
+                               // int stack[64] (src/cuda/rtKernels.cu:65-66)
+constexpr int kMaxDepth = 62;  // deepest tree for which the reference never
+                               // drops a push (stackPtr + 2 <= 64)
+
+// One sigma value (mesh, f, theta, phi) ready for the device. Frame and
+// physics constants are in the working precision R; kd / twoPid carry the
+// FP64 values used by the mixed-precision phase.
+template <typename R> struct alignas(16) Job {
+    R llc[3], uv[3], vv[3], dir[3];  // launch frame, sweep.cu:349-364
+    R k, pre, twoPi, refl;           // k, (k*dA/4pi)*2, 2pi, reflection const
+    double kd, twoPid;
+    int nx, ny, maxB, mesh;
+    int i0, j0, tilesX, tilesY;      // culled ray rectangle in tiles
+    int64_t tileBase;                // first global tile of this job
+    int nTiles;
+    int pad;
+};
+
+template <typename R> struct alignas(16) Mesh {
+    const void* nodes;
+    const void* tris;
+    R rootLo[3], rootHi[3];
+    int rootRef;
+    int nTris;
+};
+
+// Kernel arguments (passed by value).
+template <typename R> struct TraceArgs {
+    const Job<R>* jobs;
+    const Mesh<R>* meshes;
+    int nJobs;
+    int tileSide;            // rays per tile side (8, 16 or 32)
+    int64_t totalTiles;
+    int64_t tileOffset;      // first tile of a non-persistent launch
+    void* partials;          // one accumulator per tile
+    double* jobOut;          // 2 doubles per job
+    unsigned* jobDone;       // tiles finished per job
+    unsigned long long* tileCounter;  // persistent queue head
+    // per-ray records (single-job debug launches)
+    int32_t* recIds;
+    int32_t* recCount;
+    double* recL;
+    double* recReIm;
+    // work counters: rays, visits, tests, bounces, queries
+    unsigned long long* counters;
+};
+
+}  // namespace sbr

@@ -1,0 +1,527 @@
+// HIP kernels of the SBR ray path for gfx950 (MI355X).
+//
+// One fused kernel replaces the reference's two per-angle launches
+// (launchRaysMultiBounce, src/cuda/rtKernels.cu:105-158, and
+// integratePoMultiBounce, src/cuda/poKernels.cu:66-122):
+//   ray generation in registers -> closest-hit BVH traversal with
+//   Moller-Trumbore -> multi-bounce mirror loop -> PO term in registers ->
+//   wavefront shuffle reduction -> per-tile partial -> the last tile of each
+//   job reduces that job's partials in a fixed order (deterministic, no
+//   float atomics, no per-ray buffers in HBM).
+//
+// Work unit: a tile of tileSide x tileSide parallel rays of one job (one
+// sigma value). Each 256-thread workgroup is 4 wavefronts; a wavefront owns
+// an 8x8 ray patch so its 64 rays are spatially coherent.
+//
+// Traversal decisions are those of the reference's stack DFS
+// (rtKernels.cu:63-100): the BVH is stored with both children's boxes in
+// the parent record, the right child is entered first, the left child is
+// deferred on an LDS stack together with its slab entry distance, and on
+// pop it is kept iff entry <= closest -- exactly the reference's
+// "tmax >= tmin" test at the time the reference pops that node. Triangles
+// are accepted with t <= closest, so the last-tested triangle wins a tie
+// as in rtKernels.cu:36,85-88.
+//
+// Arithmetic: FP32 code uses fmaf at fixed places (dot, cross, ray origin,
+// hit point, reflection, offset); FP64 code uses no fused operations. The
+// file is compiled with -ffp-contract=off so nothing else is fused. Division
+// and square root are IEEE correctly rounded.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "sbr_device.hpp"
+
+namespace sbr {
+namespace {
+
+template <typename R> struct V3 {
+    R x, y, z;
+};
+
+template <typename R> __device__ __forceinline__ V3<R> sub(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+__device__ __forceinline__ float mn(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double mn(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float mx(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double mx(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float rsqrt_ieee(float a) { return sqrtf(a); }
+__device__ __forceinline__ double rsqrt_ieee(double a) { return sqrt(a); }
+__device__ __forceinline__ float absr(float a) { return fabsf(a); }
+__device__ __forceinline__ double absr(double a) { return fabs(a); }
+__device__ __forceinline__ float fmodr(float a, float b) { return fmodf(a, b); }
+__device__ __forceinline__ double fmodr(double a, double b) { return fmod(a, b); }
+__device__ __forceinline__ void sincosr(float a, float* s, float* c) { sincosf(a, s, c); }
+__device__ __forceinline__ void sincosr(double a, double* s, double* c) { sincos(a, s, c); }
+
+template <typename R> __device__ __forceinline__ R dot3(V3<R> a, V3<R> b) {
+    if constexpr (std::is_same_v<R, float>)
+        return fmaf(a.z, b.z, fmaf(a.x, b.x, a.y * b.y));
+    else
+        return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename R> __device__ __forceinline__ V3<R> cross3(V3<R> a, V3<R> b) {
+    if constexpr (std::is_same_v<R, float>)
+        return {fmaf(a.y, b.z, -(a.z * b.y)), -fmaf(a.x, b.z, -(a.z * b.x)), fmaf(a.x, b.y, -(a.y * b.x))};
+    else
+        return {a.y * b.z - a.z * b.y, -(a.x * b.z - a.z * b.x), a.x * b.y - a.y * b.x};
+}
+// s*a + b
+template <typename R> __device__ __forceinline__ V3<R> axpy(R s, V3<R> a, V3<R> b) {
+    if constexpr (std::is_same_v<R, float>)
+        return {fmaf(s, a.x, b.x), fmaf(s, a.y, b.y), fmaf(s, a.z, b.z)};
+    else
+        return {s * a.x + b.x, s * a.y + b.y, s * a.z + b.z};
+}
+template <typename R> __device__ __forceinline__ V3<R> unit3(V3<R> v) {
+    R len = rsqrt_ieee(dot3(v, v));
+    return {v.x / len, v.y / len, v.z / len};
+}
+
+// Slab test split in two: the closest-independent part here, the
+// "entry <= closest" part at the caller (rtKernels.cu:46-61).
+template <typename R>
+__device__ __forceinline__ bool slab(R lx, R ly, R lz, R hx, R hy, R hz, V3<R> o, V3<R> inv, R& entry) {
+    R t0x = (lx - o.x) * inv.x, t0y = (ly - o.y) * inv.y, t0z = (lz - o.z) * inv.z;
+    R t1x = (hx - o.x) * inv.x, t1y = (hy - o.y) * inv.y, t1z = (hz - o.z) * inv.z;
+    R tn = mx(mx(mn(t0x, t1x), mn(t0y, t1y)), mn(t0z, t1z));
+    R tf = mn(mn(mx(t0x, t1x), mx(t0y, t1y)), mx(t0z, t1z));
+    entry = mx(tn, R(0.001));
+    return entry <= tf;
+}
+
+template <typename R> struct TriRec {
+    V3<R> v0, e1, e2;
+    int face;
+};
+
+__device__ __forceinline__ TriRec<float> load_tri(const float* tris, int i) {
+    const float4* p = reinterpret_cast<const float4*>(tris) + 3 * i;
+    float4 a = p[0], b = p[1], c = p[2];
+    return {{a.x, a.y, a.z}, {a.w, b.x, b.y}, {b.z, b.w, c.x}, __float_as_int(c.y)};
+}
+__device__ __forceinline__ TriRec<double> load_tri(const double* tris, int i) {
+    const double2* p = reinterpret_cast<const double2*>(tris) + 5 * i;
+    double2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    int face = static_cast<int>(__double_as_longlong(e.y) & 0xffffffffll);
+    return {{a.x, a.y, b.x}, {b.y, c.x, c.y}, {d.x, d.y, e.x}, face};
+}
+
+template <typename R> struct NodeRec {
+    R b[12];
+    int l, r;
+};
+__device__ __forceinline__ NodeRec<float> load_node(const float* nodes, int i) {
+    const float4* p = reinterpret_cast<const float4*>(nodes) + 4 * i;
+    float4 a = p[0], b = p[1], c = p[2];
+    int4 d = reinterpret_cast<const int4*>(p)[3];
+    return {{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w}, d.x, d.y};
+}
+__device__ __forceinline__ NodeRec<double> load_node(const double* nodes, int i) {
+    const double2* p = reinterpret_cast<const double2*>(nodes) + 8 * i;
+    NodeRec<double> n;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double2 v = p[k];
+        n.b[2 * k] = v.x;
+        n.b[2 * k + 1] = v.y;
+    }
+    int4 d = reinterpret_cast<const int4*>(p + 6)[0];
+    n.l = d.x;
+    n.r = d.y;
+    return n;
+}
+
+// Moller-Trumbore, rtKernels.cu:10-44. Accepts t in [0.001, closest].
+template <typename R>
+__device__ __forceinline__ bool mt_test(const TriRec<R>& tr, V3<R> o, V3<R> d, R closest, R& tOut) {
+    V3<R> p = cross3(d, tr.e2);
+    R det = dot3(tr.e1, p);
+    if (absr(det) < R(1.0e-6)) return false;
+    R inv = R(1.0) / det;
+    V3<R> tv = sub(o, tr.v0);
+    R u = dot3(tv, p) * inv;
+    if (u < R(0) || u > R(1)) return false;
+    V3<R> q = cross3(tv, tr.e1);
+    R v = dot3(d, q) * inv;
+    if (v < R(0) || u + v > R(1)) return false;
+    R t = dot3(tr.e2, q) * inv;
+    if (t < R(0.001) || t > closest) return false;
+    tOut = t;
+    return true;
+}
+
+struct Work {
+    unsigned visits, tests;
+};
+
+// Closest hit (rtKernels.cu:63-100). Returns the BVH position of the hit
+// triangle or -1. The deferred-left stack lives in LDS: entry k of thread
+// tid at sRef[k*kWgSize + tid] / sE[...].
+// Entries beyond SD spill to the thread's private overflow arrays (scratch);
+// the host guarantees tree depth <= kMaxDepth so sp < kMaxStack.
+template <typename R, int SD, bool CNT>
+__device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V3<R> inv, R& tHit, int* sRef, R* sE,
+                                           int* ovRef, R* ovE, int tid, Work& w) {
+    const R* nodes = static_cast<const R*>(m.nodes);
+    const R* tris = static_cast<const R*>(m.tris);
+    R closest = R(1e20);
+    int hit = -1;
+    if (CNT) w.visits += 1;
+    R e;
+    if (!slab(m.rootLo[0], m.rootLo[1], m.rootLo[2], m.rootHi[0], m.rootHi[1], m.rootHi[2], o, inv, e)) return -1;
+    int cur = m.rootRef;
+    int sp = 0;
+    for (;;) {
+        if (cur >= 0) {
+            NodeRec<R> n = load_node(nodes, cur);
+            if (CNT) w.visits += 2;
+            R eL, eR;
+            bool hL = slab(n.b[0], n.b[1], n.b[2], n.b[3], n.b[4], n.b[5], o, inv, eL) && eL <= closest;
+            bool hR = slab(n.b[6], n.b[7], n.b[8], n.b[9], n.b[10], n.b[11], o, inv, eR) && eR <= closest;
+            if (hR) {
+                if (hL) {
+                    if (sp < SD) {
+                        sRef[sp * kWgSize + tid] = n.l;
+                        sE[sp * kWgSize + tid] = eL;
+                    } else {
+                        ovRef[sp - SD] = n.l;
+                        ovE[sp - SD] = eL;
+                    }
+                    ++sp;
+                }
+                cur = n.r;
+                continue;
+            }
+            if (hL) {
+                cur = n.l;
+                continue;
+            }
+        } else {
+            int code = ~cur;
+            int first = code >> 4;
+            int cnt = (code & 15) + 1;
+            if (CNT) w.tests += cnt;
+            for (int k = 0; k < cnt; ++k) {
+                TriRec<R> tr = load_tri(tris, first + k);
+                R t;
+                if (mt_test(tr, o, d, closest, t)) {
+                    closest = t;
+                    hit = first + k;
+                }
+            }
+        }
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            R ee;
+            int rr;
+            if (sp < SD) {
+                ee = sE[sp * kWgSize + tid];
+                rr = sRef[sp * kWgSize + tid];
+            } else {
+                ee = ovE[sp - SD];
+                rr = ovRef[sp - SD];
+            }
+            if (ee <= closest) {
+                cur = rr;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    tHit = closest;
+    return hit;
+}
+
+// ---------------------------------------------------------------- sums
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Two-sum of (s1,c1) + (s2,c2): s = s1+s2 exactly split into s + err
+__device__ __forceinline__ void two_sum_add(float& s, float& c, float s2, float c2) {
+    float t = s + s2;
+    float bp = t - s;
+    float err = (s - (t - bp)) + (s2 - bp);
+    s = t;
+    c = c + c2 + err;
+}
+__device__ __forceinline__ void wave_two_sum(float& s, float& c) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        float s2 = __shfl_xor(s, off);
+        float c2 = __shfl_xor(c, off);
+        two_sum_add(s, c, s2, c2);
+    }
+}
+
+// Accumulator per precision mode. MODE 0: plain sums in R; MODE 1 (float
+// only): two-sum compensated FP32 pairs.
+template <typename R, int MODE> struct Acc;
+template <typename R> struct Acc<R, 0> {
+    R re, im;
+    static __device__ Acc zero() { return Acc{R(0), R(0)}; }
+    __device__ void add_ray(R r, R i) { re += r; im += i; }
+    __device__ void wave() { re = wave_sum(re); im = wave_sum(im); }
+    __device__ void add(const Acc& o) { re += o.re; im += o.im; }
+    __device__ double dre() const { return double(re); }
+    __device__ double dim() const { return double(im); }
+};
+template <> struct Acc<float, 1> {
+    float re, cre, im, cim;
+    static __device__ Acc zero() { return Acc{0.f, 0.f, 0.f, 0.f}; }
+    __device__ void add_ray(float r, float i) { two_sum_add(re, cre, r, 0.f); two_sum_add(im, cim, i, 0.f); }
+    __device__ void wave() { wave_two_sum(re, cre); wave_two_sum(im, cim); }
+    __device__ void add(const Acc& o) { two_sum_add(re, cre, o.re, o.cre); two_sum_add(im, cim, o.im, o.cim); }
+    __device__ double dre() const { return double(re) + double(cre); }
+    __device__ double dim() const { return double(im) + double(cim); }
+};
+
+// Workgroup sum of one Acc per thread; result valid in thread 0.
+template <typename A> __device__ __forceinline__ A wg_sum(A a, A* red, int tid) {
+    a.wave();
+    if ((tid & 63) == 0) red[tid >> 6] = a;
+    __syncthreads();
+    A r = A::zero();
+    if (tid == 0) {
+        r = red[0];
+        for (int k = 1; k < kWgSize / 64; ++k) r.add(red[k]);
+    }
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ int find_job(const int64_t* base, int stride, int n, int64_t tile) {
+    // jobs are sorted by tileBase; binary search the last job with base <= tile
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        int64_t b = *reinterpret_cast<const int64_t*>(reinterpret_cast<const char*>(base) + size_t(mid) * stride);
+        if (b <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST>
+__global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
+    __shared__ int sRef[SD * kWgSize];
+    __shared__ R sE[SD * kWgSize];
+    __shared__ Acc<R, MODE> red[kWgSize / 64];
+    __shared__ int64_t sTile;
+    __shared__ unsigned sPrev;
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    // wave -> 8x8 quadrant of a 16x16 tile, lane -> ray within the quadrant
+    const int qx = (wave & 1) * 8 + (lane & 7);
+    const int qy = (wave >> 1) * 8 + (lane >> 3);
+    unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0;
+    int ovRef[kMaxStack - SD];
+    R ovE[kMaxStack - SD];
+
+    int64_t tile = PERSIST ? 0 : a.tileOffset + blockIdx.x;
+    for (;;) {
+        if (PERSIST) {
+            if (tid == 0) sTile = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
+            __syncthreads();
+            tile = sTile;
+            __syncthreads();
+        }
+        if (tile >= a.totalTiles) break;
+
+        const int jid = find_job(&a.jobs[0].tileBase, sizeof(Job<R>), a.nJobs, tile);
+        const Job<R>& J = a.jobs[jid];
+        const Mesh<R>& M = a.meshes[J.mesh];
+        const int tl = static_cast<int>(tile - J.tileBase);
+        const int ts = a.tileSide;
+        const int tx = tl % J.tilesX, ty = tl / J.tilesX;
+
+        Acc<R, MODE> acc = Acc<R, MODE>::zero();
+        // a tile of side ts is covered by (ts/16)^2 passes of 16x16 rays
+        for (int py = 0; py < ts; py += 16) {
+            for (int px = 0; px < ts; px += 16) {
+                const int i = J.i0 + tx * ts + px + qx;
+                const int j = J.j0 + ty * ts + py + qy;
+                if (i >= J.nx || j >= J.ny) continue;
+                if (CNT) cRays++;
+                // ray grid, rtKernels.cu:120-124
+                R u = (R(i) + R(0.5)) / R(J.nx);
+                R v = (R(j) + R(0.5)) / R(J.ny);
+                V3<R> llc{J.llc[0], J.llc[1], J.llc[2]};
+                V3<R> U{J.uv[0], J.uv[1], J.uv[2]}, Vv{J.vv[0], J.vv[1], J.vv[2]};
+                V3<R> o = axpy(v, Vv, axpy(u, U, llc));
+                V3<R> d{J.dir[0], J.dir[1], J.dir[2]};
+                int count = 0;
+                R L = R(0);
+                double Ld = 0.0;
+                V3<R> nLast{0, 0, 0}, dLast{0, 0, 0};
+                const int ray = j * J.nx + i;
+                // bounce loop, rtKernels.cu:131-158
+                for (int k = 0; k < J.maxB; ++k) {
+                    V3<R> inv{R(1.0) / d.x, R(1.0) / d.y, R(1.0) / d.z};
+                    Work w{0, 0};
+                    R t;
+                    int h = closest_hit<R, SD, CNT>(M, o, d, inv, t, sRef, sE, ovRef, ovE, tid, w);
+                    if (CNT) {
+                        cVisits += w.visits;
+                        cTests += w.tests;
+                        cQueries++;
+                    }
+                    if (h < 0) break;
+                    TriRec<R> tr = load_tri(static_cast<const R*>(M.tris), h);
+                    V3<R> n = unit3(cross3(tr.e1, tr.e2));
+                    V3<R> p = axpy(t, d, o);
+                    nLast = n;
+                    dLast = d;
+                    count++;
+                    L += t;
+                    if (MODE == 1) Ld += double(t);
+                    if (REC) a.recIds[size_t(ray) * J.maxB + k] = tr.face;
+                    R s = R(2.0) * dot3(d, n);
+                    d = unit3(axpy(-s, n, d));
+                    o = axpy(R(1e-4), n, p);
+                }
+                // PO term, poKernels.cu:75-110
+                R re = 0, im = 0;
+                if (count > 0) {
+                    R refl = R(1.0);
+                    for (int b = 0; b < count; ++b) refl *= J.refl;
+                    V3<R> nd{-dLast.x, -dLast.y, -dLast.z};
+                    R cosT = dot3(nLast, nd);
+                    if (cosT > R(0)) {
+                        R ph;
+                        if constexpr (MODE == 1) {
+                            ph = static_cast<R>(fmodr((2.0 * J.kd) * Ld, J.twoPid));
+                        } else {
+                            ph = fmodr((R(2.0) * J.k) * L, J.twoPi);
+                        }
+                        R mag = (J.pre * cosT) * refl;
+                        R sn, cs;
+                        sincosr(ph, &sn, &cs);
+                        re = -mag * sn;
+                        im = mag * cs;
+                    }
+                }
+                if (CNT) cBounces += count;
+                if (REC) {
+                    for (int k = count; k < J.maxB; ++k) a.recIds[size_t(ray) * J.maxB + k] = -1;
+                    a.recCount[ray] = count;
+                    a.recL[ray] = (MODE == 1) ? Ld : double(L);
+                    a.recReIm[2 * size_t(ray)] = double(re);
+                    a.recReIm[2 * size_t(ray) + 1] = double(im);
+                }
+                acc.add_ray(re, im);
+            }
+        }
+
+        Acc<R, MODE> tsum = wg_sum(acc, red, tid);
+        using AccT = Acc<R, MODE>;
+        AccT* parts = static_cast<AccT*>(a.partials);
+        if (tid == 0) {
+            parts[tile] = tsum;
+            __threadfence();
+            sPrev = atomicAdd(&a.jobDone[jid], 1u);
+        }
+        __syncthreads();
+        if (sPrev == static_cast<unsigned>(J.nTiles - 1)) {
+            // last tile of this job: fixed-order reduction of its partials
+            __threadfence();
+            AccT s = AccT::zero();
+            for (int q = tid; q < J.nTiles; q += kWgSize) s.add(parts[J.tileBase + q]);
+            AccT tot = wg_sum(s, red, tid);
+            if (tid == 0) {
+                a.jobOut[2 * jid] = tot.dre();
+                a.jobOut[2 * jid + 1] = tot.dim();
+            }
+        }
+        if (!PERSIST) break;
+    }
+
+    if (CNT) {
+        unsigned long long c[5] = {cRays, cVisits, cTests, cBounces, cQueries};
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            unsigned long long v = wave_sum(c[q]);
+            if (lane == 0) atomicAdd(&a.counters[q], v);
+        }
+    }
+}
+
+__global__ void zero_words(unsigned* p, size_t n) {
+    size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST>
+hipError_t launch_one(const TraceArgs<R>& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST>), dim3(grid), dim3(kWgSize), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename R, int MODE, int SD>
+hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, int grid, hipStream_t s) {
+    if (rec) return launch_one<R, MODE, SD, true, false, false>(a, grid, s);
+    if (cnt) return persist ? launch_one<R, MODE, SD, false, true, true>(a, grid, s)
+                            : launch_one<R, MODE, SD, false, true, false>(a, grid, s);
+    return persist ? launch_one<R, MODE, SD, false, false, true>(a, grid, s)
+                   : launch_one<R, MODE, SD, false, false, false>(a, grid, s);
+}
+
+}  // namespace
+
+// Host entry used by the driver. sd: traversal stack entries kept in LDS
+// per ray (8, 16 or 32); deeper entries spill to scratch.
+template <typename R>
+hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, int grid,
+                        hipStream_t s) {
+    if constexpr (std::is_same_v<R, float>) {
+        if (mode == 1) {
+            if (sd <= 8) return launch_sd<float, 1, 8>(a, rec, cnt, persist, grid, s);
+            if (sd <= 16) return launch_sd<float, 1, 16>(a, rec, cnt, persist, grid, s);
+            return launch_sd<float, 1, 32>(a, rec, cnt, persist, grid, s);
+        }
+    }
+    if (sd <= 8) return launch_sd<R, 0, 8>(a, rec, cnt, persist, grid, s);
+    if (sd <= 16) return launch_sd<R, 0, 16>(a, rec, cnt, persist, grid, s);
+    return launch_sd<R, 0, 32>(a, rec, cnt, persist, grid, s);
+}
+
+template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool, bool, bool, int, hipStream_t);
+template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, int, hipStream_t);
+
+hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned grid = static_cast<unsigned>((n + 255) / 256);
+    hipLaunchKernelGGL(zero_words, dim3(grid), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+template <typename R, int MODE, int SD> int occ_one() {
+    int n = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true>), kWgSize, 0);
+    return e == hipSuccess ? n : 1;
+}
+
+// Resident workgroups per CU of the persistent kernel variant.
+template <typename R> int trace_occupancy(int mode, int sd) {
+    if constexpr (std::is_same_v<R, float>) {
+        if (mode == 1) return sd <= 8 ? occ_one<float, 1, 8>() : (sd <= 16 ? occ_one<float, 1, 16>() : occ_one<float, 1, 32>());
+    }
+    return sd <= 8 ? occ_one<R, 0, 8>() : (sd <= 16 ? occ_one<R, 0, 16>() : occ_one<R, 0, 32>());
+}
+template int trace_occupancy<float>(int, int);
+template int trace_occupancy<double>(int, int);
+
+// Size in bytes of one per-tile partial for (precision mode).
+size_t partial_bytes(bool fp64, int mode) {
+    if (fp64) return sizeof(Acc<double, 0>);
+    return mode == 1 ? sizeof(Acc<float, 1>) : sizeof(Acc<float, 0>);
+}
+
+}  // namespace sbr

@@ -1,0 +1,138 @@
+"""GPU parity of the HIP ray path against the CPU oracle (through libsbr.so).
+
+Bar:
+  * FP32 kernel vs the oracle's FP32 twin (same fused-multiply-add places):
+    per-ray hit ids, bounce counts and path lengths bit-exact.
+  * FP64 kernel vs the FP64 oracle (the reference read literally in double):
+    per-ray hit ids, bounce counts and path lengths bit-exact.
+  * FP32 kernel vs FP64 oracle: every per-ray difference is an edge/grazing
+    tie (FP64 outcome unstable under FP32-sized perturbations).
+  * sigma: FP64 kernel within 1e-9 relative of the FP64 oracle (sin/cos
+    come from different libraries); FP32 / mixed within 0.1 dB of FP64 on
+    the sphere.
+  * per-angle and batched methods give bit-identical sums; repeated runs
+    are bit-identical.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import parity_util as PU
+
+S = pytest.importorskip("sagittasbr_amd")
+from sagittasbr_amd import meshes as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+C1 = dict(theta_start=90, theta_end=90, theta_samples=1, phi_start=0, phi_end=359, phi_samples=360, grid_size=2.5,
+          n_x=250, n_y=250, max_bounces=5, freq=3e9)
+C2S = dict(theta_start=0, theta_end=90, theta_samples=91, phi_start=0, phi_end=90, phi_samples=91, grid_size=1.6,
+           n_x=267, n_y=267, max_bounces=10, freq=10e9)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = S.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def sphere(ctx):
+    v, f = M.icosphere(5)
+    return v, f, ctx.mesh(v, f), O.Scene(v, f, 0), O.Scene(v, f, 1)
+
+
+@pytest.fixture(scope="module")
+def tri_duct(ctx):
+    v, f = M.trihedral_duct_scene()
+    return v, f, ctx.mesh(v, f), O.Scene(v, f, 0), O.Scene(v, f, 1)
+
+
+def _db(z):
+    return 10 * np.log10(np.maximum(4 * np.pi * np.abs(z) ** 2, 1e-10))
+
+
+@pytest.mark.parametrize("case,angles", [("sphere", [0, 37, 200]), ("tri_duct", [0, 4140, 2790, 8280])])
+def test_per_ray_fp32_bitexact_vs_oracle_twin(ctx, sphere, tri_duct, case, angles):
+    v, f, mesh, s32, s64 = sphere if case == "sphere" else tri_duct
+    kw = C1 if case == "sphere" else C2S
+    p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    for a in angles:
+        print(f"[per-ray fp32] {case} angle {a}", flush=True)
+        g = ctx.trace_rays(mesh, p, a, S.SBR_FP32)
+        o = s32.trace_angle(op, a, fused=True)
+        assert np.array_equal(g["count"], o["count"])
+        assert np.array_equal(g["ids"], o["ids"])
+        assert np.array_equal(g["L"], o["L"])
+        # FP32 vs FP64: only ties may differ
+        o64 = s64.trace_angle(op, a)
+        mm = PU.mismatches(g["ids"], g["count"], o64["ids"], o64["count"])
+        assert len(mm) <= 2e-3 * len(g["count"])
+        ties, real, rl = PU.classify(s64, op, a, mm, g["ids"], g["count"], limit=200)
+        assert real == 0, f"non-tie mismatches at rays {rl[:10]}"
+
+
+@pytest.mark.parametrize("case,angles", [("sphere", [0, 123]), ("tri_duct", [4140, 2790])])
+def test_per_ray_fp64_bitexact_vs_oracle(ctx, sphere, tri_duct, case, angles):
+    v, f, mesh, s32, s64 = sphere if case == "sphere" else tri_duct
+    kw = C1 if case == "sphere" else C2S
+    p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    for a in angles:
+        print(f"[per-ray fp64] {case} angle {a}", flush=True)
+        g = ctx.trace_rays(mesh, p, a, S.SBR_FP64)
+        o = s64.trace_angle(op, a)
+        assert np.array_equal(g["count"], o["count"])
+        assert np.array_equal(g["ids"], o["ids"])
+        assert np.array_equal(g["L"], o["L"])
+        np.testing.assert_allclose(g["reim"], o["reim"], rtol=0, atol=1e-12 * np.abs(o["reim"]).max())
+
+
+def test_sphere_sigma_all_precisions(ctx, sphere):
+    v, f, mesh, s32, s64 = sphere
+    kw = dict(C1, phi_end=350, phi_samples=36)
+    p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    ref, _ = s64.sweep(op)
+    ref_db = _db(ref)
+    z64, _, db64 = ctx.sweep(mesh, p, S.SBR_FP64, S.SBR_BATCHED)
+    np.testing.assert_allclose(z64, ref, rtol=1e-9)
+    for prec in (S.SBR_FP32, S.SBR_MIXED):
+        za, _, dba = ctx.sweep(mesh, p, prec, S.SBR_PER_ANGLE)
+        zb, _, dbb = ctx.sweep(mesh, p, prec, S.SBR_BATCHED)
+        assert np.array_equal(za, zb), "per-angle and batched sums differ"
+        assert np.max(np.abs(dbb - ref_db)) <= 0.1
+    # repeatability
+    z2, _, _ = ctx.sweep(mesh, p, S.SBR_MIXED, S.SBR_BATCHED)
+    z3, _, _ = ctx.sweep(mesh, p, S.SBR_MIXED, S.SBR_BATCHED)
+    assert np.array_equal(z2, z3)
+
+
+def test_tri_duct_sigma_fp64_and_methods(ctx, tri_duct):
+    v, f, mesh, s32, s64 = tri_duct
+    kw = dict(C2S, theta_samples=4, phi_samples=4, theta_end=90, phi_end=90)
+    p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    ref, _ = s64.sweep(op)
+    z64, _, _ = ctx.sweep(mesh, p, S.SBR_FP64, S.SBR_PER_ANGLE)
+    np.testing.assert_allclose(z64, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+    z64b, _, _ = ctx.sweep(mesh, p, S.SBR_FP64, S.SBR_BATCHED)
+    assert np.array_equal(z64, z64b)
+    # FP32 kernel equals the oracle's FP32 twin summed in double, up to the
+    # summation order and sin/cos of the two libraries
+    r32, _ = s32.sweep(op, fused=True)
+    z32, _, _ = ctx.sweep(mesh, p, S.SBR_FP32, S.SBR_BATCHED)
+    np.testing.assert_allclose(z32, r32, rtol=2e-4, atol=2e-4 * np.abs(r32).max())
+
+
+def test_counters_match_oracle(ctx, tri_duct):
+    v, f, mesh, s32, s64 = tri_duct
+    kw = dict(C2S, theta_samples=1, theta_start=45, theta_end=45, phi_samples=1, phi_start=45, phi_end=45)
+    p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    jobs = S.sweep_jobs(p, mesh, S.SBR_FP32)
+    pl = ctx.plan(jobs, S.SBR_FP32, S.SBR_BATCHED)
+    c = pl.count()
+    o = s32.trace_angle(op, 0, fused=True)
+    culled = op.n_rays - c["rays"]
+    assert c["tri_tests"] == o["tests"]
+    assert c["node_visits"] + culled == o["visits"]
+    assert c["ray_bounces"] == int(o["count"].sum())
+    pl.close()
