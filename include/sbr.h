@@ -142,6 +142,19 @@ int sbr_bvh_info(const sbr_bvh* b, int32_t* out4);
 int sbr_bvh_export(const sbr_bvh* b, double* tris9, int32_t* perm, double* bounds6, int32_t* links4);
 int sbr_bvh_destroy(sbr_bvh* b);
 
+/* Mesh file loading, the C form of modelLoader::loadModel
+ * (include/model/modelLoader.hpp:23-28, src/modelLoader.cpp:419-448):
+ * .obj (tinyobj number and triangulation semantics), .gltf and .glb. */
+typedef struct sbr_model sbr_model;
+int sbr_model_load(const char* path, sbr_model** out);
+/* out2 = n_vertices, n_triangles */
+int sbr_model_info(const sbr_model* m, int64_t* out2);
+/* xyz: 3*n_vertices floats; idx: 3*n_triangles indices; either may be NULL. */
+int sbr_model_copy(const sbr_model* m, float* xyz, uint32_t* idx);
+int sbr_model_destroy(sbr_model* m);
+/* tinyobj's float parsing of one token (for tests): returns 0 and *out on success. */
+int sbr_obj_parse_number(const char* token, double* out);
+
 /* Angle -> launch frame of runSweep (src/app/sweep.cu:336-364) in the given
  * precision: out12 = llc, uVec, vVec, rayDir; out_deg2 = theta, phi [deg]. */
 int sbr_angle_frame(const sbr_sweep_params* p, int angle, int precision, double* out12, double* out_deg2);

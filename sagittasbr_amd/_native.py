@@ -153,6 +153,11 @@ def lib():
         "sbr_bvh_info": (C.c_int, [_P, _P]),
         "sbr_bvh_export": (C.c_int, [_P, _P, _P, _P, _P]),
         "sbr_bvh_destroy": (C.c_int, [_P]),
+        "sbr_model_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+        "sbr_model_info": (C.c_int, [_P, _P]),
+        "sbr_model_copy": (C.c_int, [_P, _P, _P]),
+        "sbr_model_destroy": (C.c_int, [_P]),
+        "sbr_obj_parse_number": (C.c_int, [C.c_char_p, f64p]),
         "sbr_angle_frame": (C.c_int, [C.POINTER(SweepParams), C.c_int, C.c_int, _P, _P]),
         "sbr_sweep_jobs": (C.c_int, [C.POINTER(SweepParams), C.c_int, C.c_int, _P, C.c_int]),
         "sbr_plan_create": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
@@ -208,6 +213,30 @@ def host_bvh(verts, faces, precision=SBR_FP32, scale=1.0, options=None):
     finally:
         lib().sbr_bvh_destroy(h)
     return dict(tris=tris, perm=perm, bounds=bounds, links=links, root=root, depth=depth)
+
+
+def load_model(path):
+    """Load .obj / .gltf / .glb like the reference's modelLoader; returns
+    (verts float32 (V,3), faces uint32 (F,3))."""
+    h = _P()
+    _check(lib().sbr_model_load(os.fsencode(path), C.byref(h)))
+    try:
+        info = np.zeros(2, np.int64)
+        _check(lib().sbr_model_info(h, _ptr(info)))
+        v = np.zeros((int(info[0]), 3), np.float32)
+        f = np.zeros((int(info[1]), 3), np.uint32)
+        _check(lib().sbr_model_copy(h, _ptr(v), _ptr(f)))
+    finally:
+        lib().sbr_model_destroy(h)
+    return v, f
+
+
+def obj_parse_number(token: str):
+    """tinyobj's number parsing of one token (None when it fails)."""
+    out = C.c_double()
+    if lib().sbr_obj_parse_number(token.encode(), C.byref(out)) != 0:
+        return None
+    return out.value
 
 
 def angle_frame(params: SweepParams, angle: int, precision=SBR_FP32):

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "bvh_build.hpp"
+#include "mesh_io.hpp"
 #include "sbr_device.hpp"
 
 namespace sbr {
@@ -633,6 +634,43 @@ int sbr_mesh_export_bvh(sbr_ctx* ctx, int mesh_id, int precision, double* bounds
     if (precision == SBR_FP64) ex(m->f64);
     else ex(m->f32);
     return 0;
+}
+
+struct sbr_model {
+    MeshData mesh;
+};
+
+int sbr_model_load(const char* path, sbr_model** out) {
+    if (!path || !out) return fail(nullptr, "null argument");
+    auto m = std::make_unique<sbr_model>();
+    std::string err;
+    if (!load_model(path, m->mesh, err)) return fail(nullptr, err.empty() ? "failed to load model" : err);
+    *out = m.release();
+    return 0;
+}
+
+int sbr_model_info(const sbr_model* m, int64_t* out2) {
+    if (!m || !out2) return fail(nullptr, "null argument");
+    out2[0] = int64_t(m->mesh.positions.size() / 3);
+    out2[1] = int64_t(m->mesh.indices.size() / 3);
+    return 0;
+}
+
+int sbr_model_copy(const sbr_model* m, float* xyz, uint32_t* idx) {
+    if (!m) return fail(nullptr, "null argument");
+    if (xyz) std::copy(m->mesh.positions.begin(), m->mesh.positions.end(), xyz);
+    if (idx) std::copy(m->mesh.indices.begin(), m->mesh.indices.begin() + (m->mesh.indices.size() / 3) * 3, idx);
+    return 0;
+}
+
+int sbr_model_destroy(sbr_model* m) {
+    delete m;
+    return 0;
+}
+
+int sbr_obj_parse_number(const char* token, double* out) {
+    if (!token || !out) return fail(nullptr, "null argument");
+    return obj_parse_number(token, token + std::strlen(token), out) ? 0 : fail(nullptr, "not a number");
 }
 
 int sbr_angle_frame(const sbr_sweep_params* p, int angle, int precision, double* out12, double* out_deg2) {
