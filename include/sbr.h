@@ -81,9 +81,9 @@ typedef struct {
 /* Launch-shape knobs chosen by the autotuner (0 = library default). */
 typedef struct {
     int wg_per_cu;   /* persistent workgroups per CU */
-    int lds_stack;   /* traversal stack entries kept in LDS per ray */
-    int tile_rays;   /* rays per tile side (tiles are square, multiple of 8) */
-    int reserved;
+    int lds_stack;   /* traversal stack entries kept in LDS per ray (8, 16, 32) */
+    int tile_rays;   /* rays per tile side: 16, 32 or 64 */
+    int use_graph;   /* 1: capture the run's launches in a hipGraph once and replay it */
 } sbr_tuning;
 
 /* Mesh / BVH statistics. */

@@ -92,7 +92,7 @@ class Job(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("wg_per_cu", C.c_int), ("lds_stack", C.c_int), ("tile_rays", C.c_int), ("reserved", C.c_int)]
+    _fields_ = [("wg_per_cu", C.c_int), ("lds_stack", C.c_int), ("tile_rays", C.c_int), ("use_graph", C.c_int)]
 
 
 class MeshInfo(C.Structure):
@@ -266,8 +266,9 @@ class Plan:
         self.h = _P()
         _check(lib().sbr_plan_create(ctx.h, C.cast(jobs, _P), self.n, precision, method, C.byref(self.h)), ctx.h)
 
-    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0):
-        _check(lib().sbr_plan_set_tuning(self.h, C.byref(Tuning(wg_per_cu, lds_stack, tile_rays, 0))), self.ctx.h)
+    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0):
+        _check(lib().sbr_plan_set_tuning(self.h, C.byref(Tuning(wg_per_cu, lds_stack, tile_rays, use_graph))),
+               self.ctx.h)
 
     def run(self):
         _check(lib().sbr_plan_run(self.h), self.ctx.h)

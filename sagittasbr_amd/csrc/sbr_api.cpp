@@ -115,6 +115,8 @@ struct sbr_plan {
     DevBuf dJobs, dPartials, dJobOut, dJobDone, dCounter, dCounters;
     std::vector<double> out;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
     bool ran = false;
 };
 
@@ -335,8 +337,17 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
 
 int default_sd(int depth) { return depth <= 16 ? 16 : 32; }
 
-template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int32_t* rIds, int32_t* rCnt, double* rL,
-                                      double* rReIm) {
+void drop_graph(sbr_plan* pl) {
+    if (pl->graphExec) (void)hipGraphExecDestroy(pl->graphExec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    pl->graphExec = nullptr;
+    pl->graph = nullptr;
+}
+
+// Enqueue one run: clear outputs/counters with fill kernels (no memset
+// nodes, so the sequence is safe to capture), then the trace launch(es).
+template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, int32_t* rIds, int32_t* rCnt, double* rL,
+                                       double* rReIm, bool timed) {
     sbr_ctx* ctx = pl->ctx;
     const int mode = pl->precision == SBR_MIXED ? 1 : 0;
     const int sd = pl->tune.lds_stack > 0 ? pl->tune.lds_stack : default_sd(pl->maxDepth);
@@ -356,13 +367,12 @@ template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int
     a.recReIm = rReIm;
     a.counters = static_cast<unsigned long long*>(pl->dCounters.p);
     hipStream_t s = ctx->stream;
-    // clear job outputs, completion counters, queue head and work counters
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobOut.p), size_t(4) * pl->nJobs, s));
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobDone.p), size_t(pl->nJobs), s));
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounter.p), 2, s));
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounters.p), 16, s));
     // the timed region covers the trace launches only
-    HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
+    if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
     if (pl->totalTiles > 0) {
         if (pl->method == SBR_BATCHED && !record) {
             int occ = trace_occupancy<R>(mode, sd);
@@ -379,7 +389,35 @@ template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int
             }
         }
     }
-    HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
+    if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
+    return 0;
+}
+
+template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int32_t* rIds, int32_t* rCnt, double* rL,
+                                      double* rReIm) {
+    sbr_ctx* ctx = pl->ctx;
+    hipStream_t s = ctx->stream;
+    if (pl->tune.use_graph && !record && !count) {
+        if (!pl->graphExec) {
+            HIP_TRY(&ctx->err, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = plan_enqueue<R>(pl, false, false, nullptr, nullptr, nullptr, nullptr, false);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return 1;
+            }
+            if (e != hipSuccess) return fail(&ctx->err, std::string("graph capture failed: ") + hipGetErrorString(e));
+            pl->graph = g;
+            HIP_TRY(&ctx->err, hipGraphInstantiate(&pl->graphExec, g, nullptr, nullptr, 0));
+        }
+        // events bracket the whole replay (fill kernels included)
+        HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
+        HIP_TRY(&ctx->err, hipGraphLaunch(pl->graphExec, s));
+        HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
+    } else {
+        if (plan_enqueue<R>(pl, record, count, rIds, rCnt, rL, rReIm, true)) return 1;
+    }
     pl->ran = true;
     return 0;
 }
@@ -735,6 +773,7 @@ int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
     if (t->lds_stack < 0 || t->wg_per_cu < 0) return fail(&pl->ctx->err, "negative tuning value");
     if (t->tile_rays != 0 && t->tile_rays != 16 && t->tile_rays != 32 && t->tile_rays != 64)
         return fail(&pl->ctx->err, "tile_rays must be 16, 32 or 64");
+    drop_graph(pl);
     pl->tune = *t;
     int ts = t->tile_rays ? t->tile_rays : 16;
     if (ts != pl->tileSide) {
@@ -809,6 +848,7 @@ int sbr_plan_destroy(sbr_plan* pl) {
     if (!pl) return 0;
     (void)hipSetDevice(pl->ctx->device);
     (void)hipStreamSynchronize(pl->ctx->stream);
+    drop_graph(pl);
     if (pl->ev0) (void)hipEventDestroy(pl->ev0);
     if (pl->ev1) (void)hipEventDestroy(pl->ev1);
     delete pl;

@@ -68,8 +68,10 @@ def test_per_ray_fp32_bitexact_vs_oracle_twin(ctx, sphere, tri_duct, case, angle
         # FP32 vs FP64: only ties may differ
         o64 = s64.trace_angle(op, a)
         mm = PU.mismatches(g["ids"], g["count"], o64["ids"], o64["count"])
-        assert len(mm) <= 2e-3 * len(g["count"])
-        ties, real, rl = PU.classify(s64, op, a, mm, g["ids"], g["count"], limit=200)
+        # grazing aspects (a grid row lying in a plate's plane at theta = 90)
+        # produce whole lines of ties; the fraction stays small
+        assert len(mm) <= 1e-2 * len(g["count"])
+        ties, real, rl = PU.classify(s64, op, a, mm, g["ids"], g["count"], limit=300)
         assert real == 0, f"non-tie mismatches at rays {rl[:10]}"
 
 
@@ -135,4 +137,53 @@ def test_counters_match_oracle(ctx, tri_duct):
     assert c["tri_tests"] == o["tests"]
     assert c["node_visits"] + culled == o["visits"]
     assert c["ray_bounces"] == int(o["count"].sum())
+    pl.close()
+
+
+def test_graph_replay_matches_eager(ctx, sphere, tri_duct):
+    """A captured sweep replayed several times, with other eager work and
+    synchronisations between replays, gives the eager result every time."""
+    v, f, mesh, s32, s64 = tri_duct
+    kw = dict(C2S, theta_samples=6, phi_samples=6)
+    p = S.SweepParams.make(**kw)
+    for method in (S.SBR_BATCHED, S.SBR_PER_ANGLE):
+        jobs = S.sweep_jobs(p, mesh, S.SBR_MIXED)
+        pl = ctx.plan(jobs, S.SBR_MIXED, method)
+        pl.run()
+        pl.wait()
+        z0, _, _ = pl.results()
+        pl.tune(use_graph=1)
+        other = ctx.plan(S.sweep_jobs(S.SweepParams.make(**C1), sphere[2], S.SBR_FP32)[:3], S.SBR_FP32)
+        for _ in range(4):
+            pl.run()
+            pl.wait()
+            z, _, _ = pl.results()
+            assert np.array_equal(z, z0)
+            other.run()
+            other.wait()
+        other.close()
+        pl.close()
+
+
+def test_tuning_variants(ctx, tri_duct):
+    v, f, mesh, s32, s64 = tri_duct
+    p = S.SweepParams.make(**dict(C2S, theta_samples=5, phi_samples=5))
+    jobs = S.sweep_jobs(p, mesh, S.SBR_FP32)
+    pl = ctx.plan(jobs, S.SBR_FP32, S.SBR_BATCHED)
+    pl.run()
+    pl.wait()
+    z0, _, _ = pl.results()
+    for sd in (8, 16, 32):
+        for wg in (1, 2, 0):
+            pl.tune(wg_per_cu=wg, lds_stack=sd)
+            pl.run()
+            pl.wait()
+            z, _, _ = pl.results()
+            assert np.array_equal(z, z0), (sd, wg)
+    for ts in (32, 64):
+        pl.tune(tile_rays=ts)
+        pl.run()
+        pl.wait()
+        z, _, _ = pl.results()
+        np.testing.assert_allclose(z, z0, rtol=1e-4, atol=1e-5 * np.abs(z0).max())
     pl.close()
