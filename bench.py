@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the baseline sample")
     ap.add_argument("--per-angle-steps", type=int, default=1)
+    ap.add_argument("--profile-run", action="store_true",
+                    help="only the timed batched sweeps (for counter collection under rocprofv3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (rocprof FETCH/WRITE_SIZE calibration)")
     return ap.parse_args()
@@ -118,9 +120,18 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = 1000.0 * wall / args.steps
     z, sigma, dbsm = plan.results()
+    if args.profile_run:
+        print(json.dumps({"profile_run": True, "ms_per_step": ms_per_step, "kernel_ms": kernel_ms}))
+        return
 
-    # work counters (separate, untimed run) -> ray-bounces and algorithmic bytes
-    cnt = plan.count()
+    # work counters (separate, untimed runs): the reference traversal's node
+    # visits and triangle tests (FP32 plain path) give the algorithmic bytes;
+    # the measured plan's own run gives its re-trace count
+    jobs32 = S.sweep_jobs(params, mesh, S.SBR_FP32)
+    ref32 = ctx.plan(jobs32, S.SBR_FP32, S.SBR_BATCHED)
+    cnt = ref32.count()
+    own = plan.count()
+    cnt["retraced"] = own["retraced"]
     alg_bytes = cnt["node_visits"] * NODE_BYTES + cnt["tri_tests"] * TRI_BYTES + OUT_BYTES * n_angles
     k_ms = float(np.mean(kernel_ms))
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
@@ -137,7 +148,6 @@ def main():
     strong = db64 > peak_db - 30.0
 
     # plain per-angle FP32 kernel on the same GPU (one launch per angle)
-    jobs32 = S.sweep_jobs(params, mesh, S.SBR_FP32)
     pa = ctx.plan(jobs32, S.SBR_FP32, S.SBR_PER_ANGLE)
     pa.run()
     pa.wait()

@@ -20,7 +20,9 @@
  *     switch, include/RT/precision.h:29-101):
  *       SBR_FP32   FP32 traversal and physics (the reference's float build)
  *       SBR_FP64   FP64 throughout (the reference's USE_DOUBLE build)
- *       SBR_MIXED  FP32 traversal, FP64 path length and phase, FP32
+ *       SBR_MIXED  FP32 traversal with error-bounded decisions; rays with an
+ *                  ambiguous decision are re-traced in FP64, so every ray has
+ *                  the FP64 outcome; FP64 path length and phase, FP32
  *                  two-sum compensated complex sum
  */
 #ifndef SBR_H
@@ -103,6 +105,7 @@ typedef struct {
     uint64_t tri_tests;   /* Moller-Trumbore tests */
     uint64_t ray_bounces; /* sum over rays of hitCount */
     uint64_t queries;     /* closest-hit queries */
+    uint64_t retraced;    /* mixed mode: rays re-traced in FP64 after an ambiguous FP32 decision */
 } sbr_counters;
 
 const char* sbr_last_error(const sbr_ctx* ctx);
@@ -187,9 +190,10 @@ int sbr_sweep(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precisio
  * triangle ids which the reference does not record). Rays are row-major
  * (idx = j*n_x + i). Outputs (any may be NULL): hit_ids[n_rays*max_bounces]
  * original face index per bounce (-1 after the last hit); bounces[n_rays];
- * path_len[n_rays]; reim_ray[2*n_rays] PO term per ray. */
+ * path_len[n_rays]; reim_ray[2*n_rays] PO term per ray; retraced[n_rays]
+ * (SBR_MIXED) 1 where the ray was re-traced in FP64. */
 int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int angle, int precision, int32_t* hit_ids,
-                   int32_t* bounces, double* path_len, double* reim_ray);
+                   int32_t* bounces, double* path_len, double* reim_ray, int8_t* retraced);
 
 #ifdef __cplusplus
 }

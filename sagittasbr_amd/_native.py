@@ -113,6 +113,7 @@ class Counters(C.Structure):
         ("tri_tests", C.c_uint64),
         ("ray_bounces", C.c_uint64),
         ("queries", C.c_uint64),
+        ("retraced", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -169,7 +170,7 @@ def lib():
         "sbr_plan_count": (C.c_int, [_P, C.POINTER(Counters)]),
         "sbr_plan_destroy": (C.c_int, [_P]),
         "sbr_sweep": (C.c_int, [_P, C.c_int, C.POINTER(SweepParams), C.c_int, C.c_int, _P, _P, _P]),
-        "sbr_trace_rays": (C.c_int, [_P, C.c_int, C.POINTER(SweepParams), C.c_int, C.c_int, _P, _P, _P, _P]),
+        "sbr_trace_rays": (C.c_int, [_P, C.c_int, C.POINTER(SweepParams), C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -369,6 +370,8 @@ class Context:
         cnt = np.zeros(nr, np.int32)
         L = np.zeros(nr, np.float64)
         reim = np.zeros(2 * nr, np.float64)
+        flag = np.zeros(nr, np.int8)
         _check(lib().sbr_trace_rays(self.h, mesh, C.byref(params), angle, precision, _ptr(ids), _ptr(cnt), _ptr(L),
-                                    _ptr(reim)), self.h)
-        return dict(ids=ids.reshape(nr, mb)[:, : params.max_bounces], count=cnt, L=L, reim=reim.reshape(nr, 2))
+                                    _ptr(reim), _ptr(flag)), self.h)
+        return dict(ids=ids.reshape(nr, mb)[:, : params.max_bounces], count=cnt, L=L, reim=reim.reshape(nr, 2),
+                    retraced=flag)

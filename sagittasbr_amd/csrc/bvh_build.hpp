@@ -45,7 +45,7 @@ template <typename R> Bvh<R> build_bvh(const std::vector<R>& soup9, const BuildO
 //   child ref <  0 : leaf, ~ref = (firstTri << 4) | (triCount - 1)
 // FP32 record: 64 B  = lmin[3] lmax[3] rmin[3] rmax[3] lref rref pad pad
 // FP64 record: 128 B = 12 doubles, lref rref, pad to 128
-// FP32 triangle: 48 B = v0.xyz e1.xyz e2.xyz faceId 0 0 (e = v - v0)
+// FP32 triangle: 48 B = v0.xyz v1.xyz v2.xyz faceId |e1|+|e2| |e1||e2|
 // FP64 triangle: 80 B = v0.xyz e1.xyz e2.xyz faceId (int in the last slot)
 template <typename R> struct GpuBvh {
     std::vector<uint8_t> nodes;

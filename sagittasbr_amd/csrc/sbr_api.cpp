@@ -15,6 +15,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "bvh_build.hpp"
@@ -26,6 +27,9 @@ template <typename R>
 hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, int grid,
                         hipStream_t s);
 hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s);
+hipError_t launch_retrace(const TraceArgs<float>& a, bool rec, bool cnt, int grid, hipStream_t s);
+hipError_t launch_finalize(const Job<double>* jobs64, int n, const unsigned long long* fix, double* out,
+                           hipStream_t s);
 size_t partial_bytes(bool fp64, int mode);
 template <typename R> int trace_occupancy(int mode, int sd);
 }  // namespace sbr
@@ -83,6 +87,7 @@ struct MeshEntry {
     std::vector<float> xyz;
     std::vector<uint32_t> idx;
     double scale = 1.0;
+    double radius = 0.0;  // largest |vertex| after scaling (launch grid is centred on the origin)
     BuildOptions opt;
     PrecMesh<float> f32;
     PrecMesh<double> f64;
@@ -111,8 +116,11 @@ struct sbr_plan {
     int64_t totalTiles = 0;
     int maxDepth = 0;
     sbr_tuning tune{0, 0, 0, 0};
-    int tileSide = 16;
+    int tileSide = 32;
     DevBuf dJobs, dPartials, dJobOut, dJobDone, dCounter, dCounters;
+    // mixed mode
+    DevBuf dJobs64, dQueue, dQueueCount, dJobFix, dRecFlag;
+    unsigned queueCap = 0;
     std::vector<double> out;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
@@ -279,12 +287,17 @@ bool cull_rect(const Job<R>& J, const R lo[3], const R hi[3], int& i0, int& i1, 
     return true;
 }
 
+constexpr float kErrSafety = 2.0f;  // safety factor on the mixed mode's forward error bounds
+
 template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
     sbr_ctx* ctx = pl->ctx;
+    const bool mixed = pl->precision == SBR_MIXED;
     std::vector<Job<R>> jobs(pl->nJobs);
+    std::vector<Job<double>> jobs64(mixed ? pl->nJobs : 0);
     pl->tileBase.assign(pl->nJobs, 0);
     pl->nTiles.assign(pl->nJobs, 0);
     int64_t base = 0;
+    int64_t rays = 0;
     int maxDepth = 0;
     const int ts = pl->tileSide;
     for (int i = 0; i < pl->nJobs; ++i) {
@@ -300,8 +313,46 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         PrecMesh<R>& pm = prec_of<R>(m);
         maxDepth = std::max(maxDepth, pm.bvh.depth);
         Job<R> J{};
-        make_frame<R>(R(j.theta_deg), R(j.phi_deg), R(j.grid_size), J.llc, J.uv, J.vv, J.dir);
-        make_phys<R>(j, J);
+        if (!mixed) {
+            make_frame<R>(R(j.theta_deg), R(j.phi_deg), R(j.grid_size), J.llc, J.uv, J.vv, J.dir);
+            make_phys<R>(j, J);
+        } else {
+            // reference FP64 job for re-traces
+            if (ensure_built<double>(ctx, m)) return 1;
+            Job<double> D{};
+            make_frame<double>(j.theta_deg, j.phi_deg, j.grid_size, D.llc, D.uv, D.vv, D.dir);
+            make_phys<double>(j, D);
+            D.nx = j.n_x;
+            D.ny = j.n_y;
+            D.maxB = j.max_bounces;
+            D.mesh = j.mesh;
+            const double rmax = std::max(1.0, std::fabs(j.reflection_const));
+            D.fixScale = std::ldexp(1.0, 36) / (std::fabs(D.pre) * std::pow(rmax, j.max_bounces) + 1e-300);
+            // FP32 job: FP64 frame moved towards the body, rounded once
+            const double off = 10.0 * j.grid_size;
+            // keep >= 1 cm (> tMin) between the moved plane and any vertex
+            double shift = off - m.radius * (1.0 + 1e-6) - std::max(1e-3 * j.grid_size, 0.01);
+            if (shift < 0) shift = 0;
+            double llc2[3];
+            for (int k = 0; k < 3; ++k) {
+                llc2[k] = D.llc[k] + shift * D.dir[k];
+                J.llc[k] = R(llc2[k]);
+                J.uv[k] = R(D.uv[k]);
+                J.vv[k] = R(D.vv[k]);
+                J.dir[k] = R(D.dir[k]);
+            }
+            J.k = R(D.k);
+            J.pre = R(D.pre);
+            J.twoPi = R(D.twoPi);
+            J.refl = R(D.refl);
+            J.kd = D.kd;
+            J.twoPid = D.twoPid;
+            J.L0 = shift;
+            J.fixScale = D.fixScale;
+            auto nrm = [](const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+            J.ex0 = float(kErrSafety * 5.9604645e-8 * 4.0 * (nrm(llc2) + nrm(D.uv) + nrm(D.vv)));
+            jobs64[i] = D;
+        }
         J.nx = j.n_x;
         J.ny = j.n_y;
         J.maxB = j.max_bounces;
@@ -319,23 +370,41 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         pl->tileBase[i] = base;
         pl->nTiles[i] = J.nTiles;
         base += J.nTiles;
+        rays += int64_t(J.nTiles) * ts * ts;
+        if (mixed) {
+            jobs64[i].tileBase = J.tileBase;
+            jobs64[i].nTiles = J.nTiles;
+        }
         jobs[i] = J;
     }
     pl->totalTiles = base;
     pl->maxDepth = maxDepth;
     if (upload_table<R>(ctx)) return 1;
-    const int mode = pl->precision == SBR_MIXED ? 1 : 0;
+    if (mixed && upload_table<double>(ctx)) return 1;
+    const int mode = mixed ? 1 : 0;
     HIP_TRY(&ctx->err, pl->dJobs.ensure(sizeof(Job<R>) * std::max(pl->nJobs, 1)));
     HIP_TRY(&ctx->err, hipMemcpy(pl->dJobs.p, jobs.data(), sizeof(Job<R>) * pl->nJobs, hipMemcpyHostToDevice));
     HIP_TRY(&ctx->err, pl->dPartials.ensure(partial_bytes(sizeof(R) == 8, mode) * std::max<int64_t>(base, 1)));
     HIP_TRY(&ctx->err, pl->dJobOut.ensure(sizeof(double) * 2 * std::max(pl->nJobs, 1)));
     HIP_TRY(&ctx->err, pl->dJobDone.ensure(sizeof(unsigned) * std::max(pl->nJobs, 1)));
     HIP_TRY(&ctx->err, pl->dCounter.ensure(sizeof(unsigned long long)));
-    HIP_TRY(&ctx->err, pl->dCounters.ensure(sizeof(unsigned long long) * 8));
+    HIP_TRY(&ctx->err, pl->dCounters.ensure(sizeof(unsigned long long) * 16));
+    if (mixed) {
+        HIP_TRY(&ctx->err, pl->dJobs64.ensure(sizeof(Job<double>) * std::max(pl->nJobs, 1)));
+        HIP_TRY(&ctx->err,
+                hipMemcpy(pl->dJobs64.p, jobs64.data(), sizeof(Job<double>) * pl->nJobs, hipMemcpyHostToDevice));
+        // re-trace queue: room for every traced ray up to 32M entries; beyond
+        // that the trace kernel re-traces inline
+        int64_t cap = std::min<int64_t>(std::max<int64_t>(rays, 1), int64_t(1) << 25);
+        pl->queueCap = unsigned(cap);
+        HIP_TRY(&ctx->err, pl->dQueue.ensure(sizeof(QEntry) * size_t(cap)));
+        HIP_TRY(&ctx->err, pl->dQueueCount.ensure(sizeof(unsigned) * 2));
+        HIP_TRY(&ctx->err, pl->dJobFix.ensure(sizeof(unsigned long long) * 2 * std::max(pl->nJobs, 1)));
+    }
     return 0;
 }
 
-int default_sd(int depth) { return depth <= 16 ? 16 : 32; }
+int default_sd(int /*depth*/) { return 8; }
 
 void drop_graph(sbr_plan* pl) {
     if (pl->graphExec) (void)hipGraphExecDestroy(pl->graphExec);
@@ -366,11 +435,26 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     a.recL = rL;
     a.recReIm = rReIm;
     a.counters = static_cast<unsigned long long*>(pl->dCounters.p);
+    const bool mixed = pl->precision == SBR_MIXED;
+    if (mixed) {
+        a.jobs64 = static_cast<const Job<double>*>(pl->dJobs64.p);
+        a.meshes64 = static_cast<const Mesh<double>*>(ctx->meshTable64.p);
+        a.queue = static_cast<QEntry*>(pl->dQueue.p);
+        a.queueCount = static_cast<unsigned*>(pl->dQueueCount.p);
+        a.queueCap = pl->queueCap;
+        a.errK = kErrSafety;
+        a.jobFix = static_cast<unsigned long long*>(pl->dJobFix.p);
+        a.recFlag = static_cast<int8_t*>(pl->dRecFlag.p);
+    }
     hipStream_t s = ctx->stream;
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobOut.p), size_t(4) * pl->nJobs, s));
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobDone.p), size_t(pl->nJobs), s));
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounter.p), 2, s));
-    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounters.p), 16, s));
+    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounters.p), 32, s));
+    if (mixed) {
+        HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dQueueCount.p), 2, s));
+        HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobFix.p), size_t(4) * pl->nJobs, s));
+    }
     // the timed region covers the trace launches only
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
     if (pl->totalTiles > 0) {
@@ -387,6 +471,13 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
                 a.tileOffset = pl->tileBase[i];
                 HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, record, count, false, pl->nTiles[i], s));
             }
+        }
+    }
+    if constexpr (std::is_same_v<R, float>) {
+        if (mixed) {
+            // FP64 re-traces of the flagged rays, then fold their fixed-point sums in
+            HIP_TRY(&ctx->err, launch_retrace(a, record, count, ctx->cuCount * 8, s));
+            HIP_TRY(&ctx->err, launch_finalize(a.jobs64, pl->nJobs, a.jobFix, a.jobOut, s));
         }
     }
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
@@ -527,6 +618,10 @@ int sbr_mesh_create(sbr_ctx* ctx, const float* xyz, int n_verts, const uint32_t*
     m->xyz.assign(xyz, xyz + size_t(n_verts) * 3);
     m->idx.assign(idx, idx + size_t(n_tris) * 3);
     m->scale = scale;
+    for (int64_t i = 0; i < int64_t(n_verts); ++i) {
+        double x = double(xyz[3 * i]) * scale, y = double(xyz[3 * i + 1]) * scale, z = double(xyz[3 * i + 2]) * scale;
+        m->radius = std::max(m->radius, std::sqrt(x * x + y * y + z * z));
+    }
     if (parse_options(opts, m->opt)) return fail(&ctx->err, g_err);
     HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
     ctx->meshes.push_back(std::move(m));
@@ -775,7 +870,7 @@ int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
         return fail(&pl->ctx->err, "tile_rays must be 16, 32 or 64");
     drop_graph(pl);
     pl->tune = *t;
-    int ts = t->tile_rays ? t->tile_rays : 16;
+    int ts = t->tile_rays ? t->tile_rays : 32;
     if (ts != pl->tileSide) {
         pl->tileSide = ts;
         if (dispatch_prepare(pl, false)) return 1;
@@ -833,14 +928,17 @@ int sbr_plan_count(sbr_plan* pl, sbr_counters* out) {
     sbr_ctx* ctx = pl->ctx;
     HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
     if (dispatch_launch(pl, false, true)) return 1;
-    unsigned long long c[8] = {0};
+    unsigned long long c[16] = {0};
     HIP_TRY(&ctx->err, hipMemcpyAsync(c, pl->dCounters.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
+    // [0..5] trace kernel: rays, visits, tests, bounces, queries, re-traced;
+    // [6..9] FP64 re-traces: visits, tests, bounces, queries
     out->rays = c[0];
-    out->node_visits = c[1];
-    out->tri_tests = c[2];
-    out->ray_bounces = c[3];
-    out->queries = c[4];
+    out->node_visits = c[1] + c[6];
+    out->tri_tests = c[2] + c[7];
+    out->ray_bounces = c[3] + c[8];
+    out->queries = c[4] + c[9];
+    out->retraced = c[5];
     return 0;
 }
 
@@ -872,7 +970,7 @@ int sbr_sweep(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precisio
 }
 
 int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int angle, int precision, int32_t* hit_ids,
-                   int32_t* bounces, double* path_len, double* reim_ray) {
+                   int32_t* bounces, double* path_len, double* reim_ray, int8_t* retraced) {
     if (!ctx || !p) return fail(nullptr, "null argument");
     const int n = p->theta_samples * p->phi_samples;
     if (angle < 0 || angle >= n) return fail(&ctx->err, "angle index out of range");
@@ -893,6 +991,8 @@ int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int ang
         HIP_TRY(&ctx->err, hipMemsetAsync(dCnt.p, 0, nr * sizeof(int32_t), ctx->stream));
         HIP_TRY(&ctx->err, hipMemsetAsync(dL.p, 0, nr * sizeof(double), ctx->stream));
         HIP_TRY(&ctx->err, hipMemsetAsync(dRe.p, 0, nr * 2 * sizeof(double), ctx->stream));
+        HIP_TRY(&ctx->err, pl->dRecFlag.ensure(nr));
+        HIP_TRY(&ctx->err, hipMemsetAsync(pl->dRecFlag.p, 0, nr, ctx->stream));
         if (dispatch_launch(pl, true, false, static_cast<int32_t*>(dIds.p), static_cast<int32_t*>(dCnt.p),
                             static_cast<double*>(dL.p), static_cast<double*>(dRe.p)))
             return 1;
@@ -902,6 +1002,7 @@ int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int ang
         if (bounces) HIP_TRY(&ctx->err, hipMemcpy(bounces, dCnt.p, nr * sizeof(int32_t), hipMemcpyDeviceToHost));
         if (path_len) HIP_TRY(&ctx->err, hipMemcpy(path_len, dL.p, nr * sizeof(double), hipMemcpyDeviceToHost));
         if (reim_ray) HIP_TRY(&ctx->err, hipMemcpy(reim_ray, dRe.p, nr * 2 * sizeof(double), hipMemcpyDeviceToHost));
+        if (retraced) HIP_TRY(&ctx->err, hipMemcpy(retraced, pl->dRecFlag.p, nr, hipMemcpyDeviceToHost));
         return 0;
     };
     rc = run();

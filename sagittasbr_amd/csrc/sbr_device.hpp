@@ -18,11 +18,13 @@ template <typename R> struct alignas(16) Job {
     R llc[3], uv[3], vv[3], dir[3];  // launch frame, sweep.cu:349-364
     R k, pre, twoPi, refl;           // k, (k*dA/4pi)*2, 2pi, reflection const
     double kd, twoPid;
+    double L0;        // mixed mode: distance the launch plane was moved towards the body
+    double fixScale;  // mixed mode: fixed-point scale of FP64 re-trace contributions
+    float ex0;        // mixed mode: initial ray position error bound [m]
     int nx, ny, maxB, mesh;
     int i0, j0, tilesX, tilesY;      // culled ray rectangle in tiles
     int64_t tileBase;                // first global tile of this job
     int nTiles;
-    int pad;
 };
 
 template <typename R> struct alignas(16) Mesh {
@@ -31,6 +33,12 @@ template <typename R> struct alignas(16) Mesh {
     R rootLo[3], rootHi[3];
     int rootRef;
     int nTris;
+};
+
+// FP64 re-trace request of the mixed mode
+struct QEntry {
+    int job;
+    int ray;
 };
 
 // Kernel arguments (passed by value).
@@ -50,8 +58,18 @@ template <typename R> struct TraceArgs {
     int32_t* recCount;
     double* recL;
     double* recReIm;
-    // work counters: rays, visits, tests, bounces, queries
+    // work counters: rays, visits, tests, bounces, queries, re-traced rays
     unsigned long long* counters;
+    // mixed mode: reference FP64 jobs and meshes for re-traces, the re-trace
+    // queue, fixed-point per-job accumulators, per-ray re-trace flags
+    const Job<double>* jobs64;
+    const Mesh<double>* meshes64;
+    QEntry* queue;
+    unsigned* queueCount;
+    unsigned queueCap;
+    float errK;
+    unsigned long long* jobFix;
+    int8_t* recFlag;
 };
 
 }  // namespace sbr

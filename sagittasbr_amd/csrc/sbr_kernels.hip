@@ -26,6 +26,17 @@
 // hit point, reflection, offset); FP64 code uses no fused operations. The
 // file is compiled with -ffp-contract=off so nothing else is fused. Division
 // and square root are IEEE correctly rounded.
+//
+// Mixed mode (MODE 1): FP32 traversal whose every relevant decision carries
+// a forward error bound against the FP64 reference ray (filtered
+// predicates). Boxes are tested grown by the ray's position error bound, so
+// pruning is conservative; Moller-Trumbore decisions whose barycentric,
+// determinant or distance margin falls inside the bound flag the ray. A
+// flagged ray contributes nothing in FP32 and is re-traced by the FP64
+// reference path (queue kernel), whose contribution is accumulated in
+// 64-bit fixed point (order independent, hence deterministic). Unflagged
+// rays therefore have the FP64 outcome; path length and phase are FP64 and
+// the complex sum is FP32 two-sum compensated.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -93,18 +104,32 @@ __device__ __forceinline__ bool slab(R lx, R ly, R lz, R hx, R hy, R hz, V3<R> o
 template <typename R> struct TriRec {
     V3<R> v0, e1, e2;
     int face;
+    float sumE, prodE;  // |e1| + |e2| and |e1| |e2| (FP32 layout only)
 };
 
+// FP32 record: v0, v1, v2, face, |e1|+|e2|, |e1||e2|; the edges are formed
+// here exactly as rtKernels.cu:13-14 forms them.
 __device__ __forceinline__ TriRec<float> load_tri(const float* tris, int i) {
     const float4* p = reinterpret_cast<const float4*>(tris) + 3 * i;
     float4 a = p[0], b = p[1], c = p[2];
-    return {{a.x, a.y, a.z}, {a.w, b.x, b.y}, {b.z, b.w, c.x}, __float_as_int(c.y)};
+    V3<float> v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
+    return {v0, sub(v1, v0), sub(v2, v0), __float_as_int(c.y), c.z, c.w};
+}
+// Exact FP64 vertices of an FP32 record (for the mixed mode's recompute).
+__device__ __forceinline__ void load_tri_verts64(const float* tris, int i, V3<double>& v0, V3<double>& e1,
+                                                 V3<double>& e2, int& face) {
+    const float4* p = reinterpret_cast<const float4*>(tris) + 3 * i;
+    float4 a = p[0], b = p[1], c = p[2];
+    v0 = {double(a.x), double(a.y), double(a.z)};
+    e1 = {double(a.w) - v0.x, double(b.x) - v0.y, double(b.y) - v0.z};
+    e2 = {double(b.z) - v0.x, double(b.w) - v0.y, double(c.x) - v0.z};
+    face = __float_as_int(c.y);
 }
 __device__ __forceinline__ TriRec<double> load_tri(const double* tris, int i) {
     const double2* p = reinterpret_cast<const double2*>(tris) + 5 * i;
     double2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
     int face = static_cast<int>(__double_as_longlong(e.y) & 0xffffffffll);
-    return {{a.x, a.y, b.x}, {b.y, c.x, c.y}, {d.x, d.y, e.x}, face};
+    return {{a.x, a.y, b.x}, {b.y, c.x, c.y}, {d.x, d.y, e.x}, face, 0.f, 0.f};
 }
 
 template <typename R> struct NodeRec {
@@ -181,7 +206,7 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
             bool hR = slab(n.b[6], n.b[7], n.b[8], n.b[9], n.b[10], n.b[11], o, inv, eR) && eR <= closest;
             if (hR) {
                 if (hL) {
-                    if (sp < SD) {
+                    if (SD > 0 && sp < SD) {
                         sRef[sp * kWgSize + tid] = n.l;
                         sE[sp * kWgSize + tid] = eL;
                     } else {
@@ -216,7 +241,7 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
             --sp;
             R ee;
             int rr;
-            if (sp < SD) {
+            if (SD > 0 && sp < SD) {
                 ee = sE[sp * kWgSize + tid];
                 rr = sRef[sp * kWgSize + tid];
             } else {
@@ -230,6 +255,140 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
             }
         }
         if (!found) break;
+    }
+    tHit = closest;
+    return hit;
+}
+
+// ------------------------------------------------- mixed-mode traversal
+constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
+
+template <int SD, bool CNT>
+__device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> d, V3<float> inv,
+                                                 float& tHit, int* sRef, float* sE, int* ovRef, float* ovE, int tid,
+                                                 Work& w, float ex, float ed, float K, bool& amb) {
+    const float* nodes = static_cast<const float*>(m.nodes);
+    const float* tris = static_cast<const float*>(m.tris);
+    const float o1 = fabsf(o.x) + fabsf(o.y) + fabsf(o.z);
+    // far distance of anything in the root box from the origin
+    float cx = 0.5f * (m.rootLo[0] + m.rootHi[0]), cy = 0.5f * (m.rootLo[1] + m.rootHi[1]),
+          cz = 0.5f * (m.rootLo[2] + m.rootHi[2]);
+    float hx = m.rootHi[0] - cx, hy = m.rootHi[1] - cy, hz = m.rootHi[2] - cz;
+    float tFar = sqrtf((o.x - cx) * (o.x - cx) + (o.y - cy) * (o.y - cy) + (o.z - cz) * (o.z - cz)) +
+                 sqrtf(hx * hx + hy * hy + hz * hz);
+    // position and distance error bounds valid over the whole query
+    float dx = ex + ed * tFar + K * kEps32 * (o1 + tFar);
+    float tErr = dx + 4.0f * K * kEps32 * tFar;
+    const float tMinE = 0.001f - tErr;
+    float closest = 1e20f, tBestErr = 0.0f;
+    float tAmb = 3.0e38f, t2 = 3.0e38f;
+    int hit = -1;
+    float e;
+    if (CNT) w.visits += 1;
+    if (!slab(m.rootLo[0] - dx, m.rootLo[1] - dx, m.rootLo[2] - dx, m.rootHi[0] + dx, m.rootHi[1] + dx,
+              m.rootHi[2] + dx, o, inv, e)) {
+        tHit = closest;
+        return -1;
+    }
+    (void)tMinE;
+    int cur = m.rootRef;
+    int sp = 0;
+    for (;;) {
+        const float lim = closest + tErr;
+        if (cur >= 0) {
+            NodeRec<float> n = load_node(nodes, cur);
+            if (CNT) w.visits += 2;
+            float eL, eR;
+            bool hL = slab(n.b[0] - dx, n.b[1] - dx, n.b[2] - dx, n.b[3] + dx, n.b[4] + dx, n.b[5] + dx, o, inv, eL) &&
+                      eL <= lim;
+            bool hR = slab(n.b[6] - dx, n.b[7] - dx, n.b[8] - dx, n.b[9] + dx, n.b[10] + dx, n.b[11] + dx, o, inv,
+                           eR) &&
+                      eR <= lim;
+            if (hR) {
+                if (hL) {
+                    if (sp < SD) {
+                        sRef[sp * kWgSize + tid] = n.l;
+                        sE[sp * kWgSize + tid] = eL;
+                    } else {
+                        ovRef[sp - SD] = n.l;
+                        ovE[sp - SD] = eL;
+                    }
+                    ++sp;
+                }
+                cur = n.r;
+                continue;
+            }
+            if (hL) {
+                cur = n.l;
+                continue;
+            }
+        } else {
+            int code = ~cur;
+            int first = code >> 4;
+            int cnt = (code & 15) + 1;
+            if (CNT) w.tests += cnt;
+            for (int k = 0; k < cnt; ++k) {
+                TriRec<float> tr = load_tri(tris, first + k);
+                V3<float> p = cross3(d, tr.e2);
+                float det = dot3(tr.e1, p);
+                float adet = fabsf(det);
+                float invd = 1.0f / det;
+                V3<float> tv = sub(o, tr.v0);
+                float u = dot3(tv, p) * invd;
+                V3<float> q = cross3(tv, tr.e1);
+                float v = dot3(d, q) * invd;
+                float t = dot3(tr.e2, q) * invd;
+                // forward error bounds of this test against the FP64 ray
+                float at = fabsf(t);
+                float v01 = fabsf(tr.v0.x) + fabsf(tr.v0.y) + fabsf(tr.v0.z);
+                float xerr = ex + ed * at + K * kEps32 * (o1 + v01 + 1.0f);
+                float rdet = 1.0f / fmaxf(adet, 1e-30f);
+                float berr = xerr * tr.sumE * rdet + 4.0f * K * kEps32;
+                float terr = xerr * tr.prodE * rdet + K * kEps32 * (o1 + v01 + at);
+                float ddet = (ed + K * kEps32) * tr.prodE * 2.0f;
+                float mrg = fminf(u, fminf(v, 1.0f - u - v));
+                bool detOk = adet >= 1.0e-6f;
+                bool inside = detOk && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || u + v > 1.0f);
+                bool nearEdge = detOk && fabsf(mrg) <= berr;
+                bool ambig = (fabsf(adet - 1.0e-6f) <= ddet) || nearEdge ||
+                             ((inside || nearEdge) && fabsf(t - 0.001f) <= terr);
+                if (ambig && t >= 0.001f - terr) tAmb = fminf(tAmb, t - terr);
+                if (inside && t >= 0.001f && t <= closest) {
+                    if (closest < 1e19f) t2 = fminf(t2, closest);
+                    closest = t;
+                    tBestErr = terr;
+                    hit = first + k;
+                } else if (inside && t > closest) {
+                    t2 = fminf(t2, t);
+                }
+            }
+        }
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            float ee;
+            int rr;
+            if (sp < SD) {
+                ee = sE[sp * kWgSize + tid];
+                rr = sRef[sp * kWgSize + tid];
+            } else {
+                ee = ovE[sp - SD];
+                rr = ovRef[sp - SD];
+            }
+            if (ee <= closest + tErr) {
+                cur = rr;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    if (hit < 0) {
+        if (tAmb < 3.0e38f) amb = true;
+    } else {
+        float tol = 2.0f * tBestErr + 1.0e-6f * closest;
+        if (tAmb <= closest + tol) amb = true;
+        if (t2 - closest <= tol) amb = true;
     }
     tHit = closest;
     return hit;
@@ -260,7 +419,7 @@ __device__ __forceinline__ void wave_two_sum(float& s, float& c) {
 }
 
 // Accumulator per precision mode. MODE 0: plain sums in R; MODE 1 (float
-// only): two-sum compensated FP32 pairs.
+// only): two-sum compensated FP32 (sum, error) pairs.
 template <typename R, int MODE> struct Acc;
 template <typename R> struct Acc<R, 0> {
     R re, im;
@@ -275,6 +434,12 @@ template <> struct Acc<float, 1> {
     float re, cre, im, cim;
     static __device__ Acc zero() { return Acc{0.f, 0.f, 0.f, 0.f}; }
     __device__ void add_ray(float r, float i) { two_sum_add(re, cre, r, 0.f); two_sum_add(im, cim, i, 0.f); }
+    // an FP64 term enters as its FP32 (head, tail) pair
+    __device__ void add_ray64(double r, double i) {
+        float rh = float(r), ih = float(i);
+        two_sum_add(re, cre, rh, float(r - double(rh)));
+        two_sum_add(im, cim, ih, float(i - double(ih)));
+    }
     __device__ void wave() { wave_two_sum(re, cre); wave_two_sum(im, cim); }
     __device__ void add(const Acc& o) { two_sum_add(re, cre, o.re, o.cre); two_sum_add(im, cim, o.im, o.cim); }
     __device__ double dre() const { return double(re) + double(cre); }
@@ -307,6 +472,152 @@ __device__ __forceinline__ int find_job(const int64_t* base, int stride, int n, 
     return lo;
 }
 
+// ------------------------------------------------------------ ray paths
+template <typename R> struct PathOut {
+    int count;
+    R L;
+    double Ld;
+    V3<R> n, d;
+};
+
+// Reference ray (rtKernels.cu:113-158): generation, bounces, path length.
+// SD == 0 keeps the whole traversal stack in private memory.
+template <typename R, int SD, bool CNT, bool REC>
+__device__ __forceinline__ PathOut<R> trace_path(const Job<R>& J, const Mesh<R>& M, int i, int j, int* sRef, R* sE,
+                                                 int* ovRef, R* ovE, int tid, int32_t* ids, Work& w,
+                                                 unsigned long long& queries) {
+    R u = (R(i) + R(0.5)) / R(J.nx);
+    R v = (R(j) + R(0.5)) / R(J.ny);
+    V3<R> llc{J.llc[0], J.llc[1], J.llc[2]};
+    V3<R> U{J.uv[0], J.uv[1], J.uv[2]}, Vv{J.vv[0], J.vv[1], J.vv[2]};
+    V3<R> o = axpy(v, Vv, axpy(u, U, llc));
+    V3<R> d{J.dir[0], J.dir[1], J.dir[2]};
+    PathOut<R> out{0, R(0), 0.0, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < J.maxB; ++k) {
+        V3<R> inv{R(1.0) / d.x, R(1.0) / d.y, R(1.0) / d.z};
+        R t;
+        int h = closest_hit<R, SD, CNT>(M, o, d, inv, t, sRef, sE, ovRef, ovE, tid, w);
+        if (CNT) queries++;
+        if (h < 0) break;
+        TriRec<R> tr = load_tri(static_cast<const R*>(M.tris), h);
+        V3<R> n = unit3(cross3(tr.e1, tr.e2));
+        V3<R> p = axpy(t, d, o);
+        out.n = n;
+        out.d = d;
+        out.count++;
+        out.L += t;
+        if (REC) ids[k] = tr.face;
+        R s = R(2.0) * dot3(d, n);
+        d = unit3(axpy(-s, n, d));
+        o = axpy(R(1e-4), n, p);
+    }
+    out.Ld = double(out.L);
+    return out;
+}
+
+// PO term (poKernels.cu:75-110) from a finished path; the phase uses the
+// FP64 path length when fp64Phase is set.
+template <typename R>
+__device__ __forceinline__ void po_term(const Job<R>& J, const PathOut<R>& P, bool fp64Phase, R& re, R& im) {
+    re = R(0);
+    im = R(0);
+    if (P.count <= 0) return;
+    R refl = R(1.0);
+    for (int b = 0; b < P.count; ++b) refl *= J.refl;
+    V3<R> nd{-P.d.x, -P.d.y, -P.d.z};
+    R cosT = dot3(P.n, nd);
+    if (!(cosT > R(0))) return;
+    R ph;
+    if (fp64Phase) ph = static_cast<R>(fmodr((2.0 * J.kd) * P.Ld, J.twoPid));
+    else ph = fmodr((R(2.0) * J.k) * P.L, J.twoPi);
+    R mag = (J.pre * cosT) * refl;
+    R sn, cs;
+    sincosr(ph, &sn, &cs);
+    re = -mag * sn;
+    im = mag * cs;
+}
+
+// Mixed-mode ray. The ray state (origin, direction, path length, last
+// normal) is carried in FP64 with the reference's FP64 operations, so it is
+// the FP64 reference ray itself; each closest-hit query runs on an FP32 copy
+// of it (the first one from a launch plane moved next to the body) with
+// error-bounded decisions, and the chosen triangle's distance is then
+// recomputed in FP64. Returns false when a query was ambiguous.
+template <int SD, bool CNT, bool REC>
+__device__ __forceinline__ bool trace_path_mixed(const Job<float>& J, const Job<double>& D, const Mesh<float>& M,
+                                                 int i, int j, int* sRef, float* sE, int* ovRef, float* ovE, int tid,
+                                                 int32_t* ids, Work& w, unsigned long long& queries, float K,
+                                                 PathOut<double>& out) {
+    double u = (double(i) + 0.5) / double(D.nx);
+    double v = (double(j) + 0.5) / double(D.ny);
+    V3<double> llc{D.llc[0], D.llc[1], D.llc[2]};
+    V3<double> U{D.uv[0], D.uv[1], D.uv[2]}, Vv{D.vv[0], D.vv[1], D.vv[2]};
+    V3<double> o = axpy(v, Vv, axpy(u, U, llc));
+    V3<double> d{D.dir[0], D.dir[1], D.dir[2]};
+    out = PathOut<double>{0, 0.0, 0.0, {0, 0, 0}, {0, 0, 0}};
+    const float* tris = static_cast<const float*>(M.tris);
+    for (int k = 0; k < J.maxB; ++k) {
+        // FP32 copy of the ray; the launch-plane move only shifts the
+        // origin along the ray, which changes no decision
+        const double shift = k == 0 ? J.L0 : 0.0;
+        V3<float> of{float(o.x + shift * d.x), float(o.y + shift * d.y), float(o.z + shift * d.z)};
+        V3<float> df{float(d.x), float(d.y), float(d.z)};
+        V3<float> inv{1.0f / df.x, 1.0f / df.y, 1.0f / df.z};
+        const float ex = K * kEps32 * (fabsf(of.x) + fabsf(of.y) + fabsf(of.z) + 1e-3f);
+        const float ed = 2.0f * K * kEps32;
+        float tf;
+        bool amb = false;
+        int h = closest_hit_mixed<SD, CNT>(M, of, df, inv, tf, sRef, sE, ovRef, ovE, tid, w, ex, ed, K, amb);
+        if (CNT) queries++;
+        if (amb) return false;
+        if (h < 0) break;
+        // FP64 recompute of the hit on the chosen triangle (rtKernels.cu:13-41)
+        V3<double> v0, e1, e2;
+        int face;
+        load_tri_verts64(tris, h, v0, e1, e2, face);
+        V3<double> p = cross3(d, e2);
+        double det = dot3(e1, p);
+        double invd = 1.0 / det;
+        V3<double> tv = sub(o, v0);
+        V3<double> q = cross3(tv, e1);
+        double t = dot3(e2, q) * invd;
+        V3<double> n = unit3(cross3(e1, e2));
+        V3<double> hp = axpy(t, d, o);
+        out.n = n;
+        out.d = d;
+        out.count++;
+        out.L += t;
+        if (REC) ids[k] = face;
+        double s = 2.0 * dot3(d, n);
+        d = unit3(axpy(-s, n, d));
+        o = axpy(1e-4, n, hp);
+    }
+    out.Ld = out.L;
+    return true;
+}
+
+// Reference FP64 ray with a private stack (mixed-mode re-traces); writes
+// records when ids != nullptr.
+template <bool CNT>
+__device__ __forceinline__ void fp64_retrace(const Job<double>& J, const Mesh<double>& M, int i, int j, int32_t* ids,
+                                             Work& w, unsigned long long& queries, PathOut<double>& P, double& re,
+                                             double& im) {
+    int ovRef[kMaxStack];
+    double ovE[kMaxStack];
+    if (ids) {
+        P = trace_path<double, 0, CNT, true>(J, M, i, j, nullptr, nullptr, ovRef, ovE, 0, ids, w, queries);
+        for (int k = P.count; k < J.maxB; ++k) ids[k] = -1;
+    } else {
+        P = trace_path<double, 0, CNT, false>(J, M, i, j, nullptr, nullptr, ovRef, ovE, 0, nullptr, w, queries);
+    }
+    po_term<double>(J, P, false, re, im);
+}
+
+__device__ __forceinline__ void fix_add(unsigned long long* p, double v, double scale) {
+    long long q = __double2ll_rn(v * scale);
+    atomicAdd(p, static_cast<unsigned long long>(q));
+}
+
 template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST>
 __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
     __shared__ int sRef[SD * kWgSize];
@@ -317,10 +628,10 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    // wave -> 8x8 quadrant of a 16x16 tile, lane -> ray within the quadrant
+    // wave -> 8x8 quadrant of a 16x16 ray pass, lane -> ray within the quadrant
     const int qx = (wave & 1) * 8 + (lane & 7);
     const int qy = (wave >> 1) * 8 + (lane >> 3);
-    unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0;
+    unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0, cRetrace = 0;
     int ovRef[kMaxStack - SD];
     R ovE[kMaxStack - SD];
 
@@ -349,73 +660,67 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
                 const int j = J.j0 + ty * ts + py + qy;
                 if (i >= J.nx || j >= J.ny) continue;
                 if (CNT) cRays++;
-                // ray grid, rtKernels.cu:120-124
-                R u = (R(i) + R(0.5)) / R(J.nx);
-                R v = (R(j) + R(0.5)) / R(J.ny);
-                V3<R> llc{J.llc[0], J.llc[1], J.llc[2]};
-                V3<R> U{J.uv[0], J.uv[1], J.uv[2]}, Vv{J.vv[0], J.vv[1], J.vv[2]};
-                V3<R> o = axpy(v, Vv, axpy(u, U, llc));
-                V3<R> d{J.dir[0], J.dir[1], J.dir[2]};
-                int count = 0;
-                R L = R(0);
-                double Ld = 0.0;
-                V3<R> nLast{0, 0, 0}, dLast{0, 0, 0};
                 const int ray = j * J.nx + i;
-                // bounce loop, rtKernels.cu:131-158
-                for (int k = 0; k < J.maxB; ++k) {
-                    V3<R> inv{R(1.0) / d.x, R(1.0) / d.y, R(1.0) / d.z};
-                    Work w{0, 0};
-                    R t;
-                    int h = closest_hit<R, SD, CNT>(M, o, d, inv, t, sRef, sE, ovRef, ovE, tid, w);
-                    if (CNT) {
-                        cVisits += w.visits;
-                        cTests += w.tests;
-                        cQueries++;
-                    }
-                    if (h < 0) break;
-                    TriRec<R> tr = load_tri(static_cast<const R*>(M.tris), h);
-                    V3<R> n = unit3(cross3(tr.e1, tr.e2));
-                    V3<R> p = axpy(t, d, o);
-                    nLast = n;
-                    dLast = d;
-                    count++;
-                    L += t;
-                    if (MODE == 1) Ld += double(t);
-                    if (REC) a.recIds[size_t(ray) * J.maxB + k] = tr.face;
-                    R s = R(2.0) * dot3(d, n);
-                    d = unit3(axpy(-s, n, d));
-                    o = axpy(R(1e-4), n, p);
-                }
-                // PO term, poKernels.cu:75-110
+                int32_t* ids = REC ? a.recIds + size_t(ray) * J.maxB : nullptr;
+                Work w{0, 0};
                 R re = 0, im = 0;
-                if (count > 0) {
-                    R refl = R(1.0);
-                    for (int b = 0; b < count; ++b) refl *= J.refl;
-                    V3<R> nd{-dLast.x, -dLast.y, -dLast.z};
-                    R cosT = dot3(nLast, nd);
-                    if (cosT > R(0)) {
-                        R ph;
-                        if constexpr (MODE == 1) {
-                            ph = static_cast<R>(fmodr((2.0 * J.kd) * Ld, J.twoPid));
+                double dre = 0.0, dimg = 0.0;
+                PathOut<R> P;
+                PathOut<double> PD;
+                bool ok = true;
+                if constexpr (MODE == 1) {
+                    const Job<double>& D = a.jobs64[jid];
+                    ok = trace_path_mixed<SD, CNT, REC>(J, D, M, i, j, sRef, sE, ovRef, ovE, tid, ids, w, cQueries,
+                                                       a.errK, PD);
+                    if (ok) po_term<double>(D, PD, false, dre, dimg);
+                    P.count = PD.count;
+                    P.Ld = PD.Ld;
+                } else {
+                    P = trace_path<R, SD, CNT, REC>(J, M, i, j, sRef, sE, ovRef, ovE, tid, ids, w, cQueries);
+                    po_term<R>(J, P, false, re, im);
+                }
+                if (CNT) {
+                    cVisits += w.visits;
+                    cTests += w.tests;
+                }
+                if constexpr (MODE == 1) {
+                    if (!ok) {
+                        if (CNT) cRetrace++;
+                        if (REC) a.recFlag[ray] = 1;
+                        unsigned q = atomicAdd(a.queueCount, 1u);
+                        if (q < a.queueCap) {
+                            a.queue[q] = QEntry{jid, ray};
                         } else {
-                            ph = fmodr((R(2.0) * J.k) * L, J.twoPi);
+                            // queue full: re-trace inline (same FP64 path, same fixed-point sum)
+                            const Job<double>& J64 = a.jobs64[jid];
+                            PathOut<double> P64;
+                            double r64, i64;
+                            Work w2{0, 0};
+                            unsigned long long q2 = 0;
+                            fp64_retrace<false>(J64, a.meshes64[J64.mesh], i, j, REC ? ids : nullptr, w2, q2, P64,
+                                                r64, i64);
+                            fix_add(&a.jobFix[2 * jid], r64, J64.fixScale);
+                            fix_add(&a.jobFix[2 * jid + 1], i64, J64.fixScale);
+                            if (REC) {
+                                a.recCount[ray] = P64.count;
+                                a.recL[ray] = P64.Ld;
+                                a.recReIm[2 * size_t(ray)] = r64;
+                                a.recReIm[2 * size_t(ray) + 1] = i64;
+                            }
                         }
-                        R mag = (J.pre * cosT) * refl;
-                        R sn, cs;
-                        sincosr(ph, &sn, &cs);
-                        re = -mag * sn;
-                        im = mag * cs;
+                        continue;
                     }
                 }
-                if (CNT) cBounces += count;
+                if (CNT) cBounces += P.count;
                 if (REC) {
-                    for (int k = count; k < J.maxB; ++k) a.recIds[size_t(ray) * J.maxB + k] = -1;
-                    a.recCount[ray] = count;
-                    a.recL[ray] = (MODE == 1) ? Ld : double(L);
-                    a.recReIm[2 * size_t(ray)] = double(re);
-                    a.recReIm[2 * size_t(ray) + 1] = double(im);
+                    for (int k = P.count; k < J.maxB; ++k) ids[k] = -1;
+                    a.recCount[ray] = P.count;
+                    a.recL[ray] = MODE == 1 ? P.Ld : double(P.L);
+                    a.recReIm[2 * size_t(ray)] = MODE == 1 ? dre : double(re);
+                    a.recReIm[2 * size_t(ray) + 1] = MODE == 1 ? dimg : double(im);
                 }
-                acc.add_ray(re, im);
+                if constexpr (MODE == 1) acc.add_ray64(dre, dimg);
+                else acc.add_ray(re, im);
             }
         }
 
@@ -443,13 +748,61 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
     }
 
     if (CNT) {
-        unsigned long long c[5] = {cRays, cVisits, cTests, cBounces, cQueries};
+        unsigned long long c[6] = {cRays, cVisits, cTests, cBounces, cQueries, cRetrace};
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
+        for (int q = 0; q < 6; ++q) {
             unsigned long long v = wave_sum(c[q]);
             if (lane == 0) atomicAdd(&a.counters[q], v);
         }
     }
+}
+
+// FP64 re-traces of the mixed mode's flagged rays (grid-stride over the queue).
+template <bool REC, bool CNT>
+__global__ __launch_bounds__(kWgSize) void retrace_kernel(TraceArgs<float> a) {
+    const unsigned n = min(*a.queueCount, a.queueCap);
+    unsigned long long cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0;
+    for (unsigned idx = blockIdx.x * kWgSize + threadIdx.x; idx < n; idx += gridDim.x * kWgSize) {
+        QEntry e = a.queue[idx];
+        const Job<double>& J = a.jobs64[e.job];
+        const int i = e.ray % J.nx, j = e.ray / J.nx;
+        Work w{0, 0};
+        PathOut<double> P;
+        double re, im;
+        fp64_retrace<CNT>(J, a.meshes64[J.mesh], i, j, REC ? a.recIds + size_t(e.ray) * J.maxB : nullptr, w,
+                          cQueries, P, re, im);
+        fix_add(&a.jobFix[2 * e.job], re, J.fixScale);
+        fix_add(&a.jobFix[2 * e.job + 1], im, J.fixScale);
+        if (CNT) {
+            cVisits += w.visits;
+            cTests += w.tests;
+            cBounces += P.count;
+        }
+        if (REC) {
+            a.recCount[e.ray] = P.count;
+            a.recL[e.ray] = P.Ld;
+            a.recReIm[2 * size_t(e.ray)] = re;
+            a.recReIm[2 * size_t(e.ray) + 1] = im;
+        }
+    }
+    if (CNT) {
+        unsigned long long c[4] = {cVisits, cTests, cBounces, cQueries};
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            unsigned long long v = wave_sum(c[q]);
+            if (lane == 0) atomicAdd(&a.counters[6 + q], v);
+        }
+    }
+}
+
+// Adds the fixed-point FP64 re-trace sums to the job outputs.
+__global__ void finalize_kernel(const Job<double>* jobs64, int n, const unsigned long long* fix, double* out) {
+    int jid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jid >= n) return;
+    const double inv = 1.0 / jobs64[jid].fixScale;
+    out[2 * jid] += double(static_cast<long long>(fix[2 * jid])) * inv;
+    out[2 * jid + 1] += double(static_cast<long long>(fix[2 * jid + 1])) * inv;
 }
 
 __global__ void zero_words(unsigned* p, size_t n) {
@@ -493,6 +846,20 @@ hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool 
 
 template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool, bool, bool, int, hipStream_t);
 template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, int, hipStream_t);
+
+hipError_t launch_retrace(const TraceArgs<float>& a, bool rec, bool cnt, int grid, hipStream_t s) {
+    if (rec) hipLaunchKernelGGL((retrace_kernel<true, false>), dim3(grid), dim3(kWgSize), 0, s, a);
+    else if (cnt) hipLaunchKernelGGL((retrace_kernel<false, true>), dim3(grid), dim3(kWgSize), 0, s, a);
+    else hipLaunchKernelGGL((retrace_kernel<false, false>), dim3(grid), dim3(kWgSize), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const Job<double>* jobs64, int n, const unsigned long long* fix, double* out,
+                           hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, s, jobs64, n, fix, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
