@@ -460,6 +460,16 @@ template <typename A> __device__ __forceinline__ A wg_sum(A a, A* red, int tid) 
     return r;
 }
 
+// Values every lane has just read from LDS after a barrier, handed to the
+// compiler as wave-uniform scalars so that the loop exit and the branches
+// around barriers that depend on them are uniform branches.
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+    unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<uint64_t>(v)));
+    unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<uint64_t>(v) >> 32));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ unsigned uniform32(unsigned v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ int find_job(const int64_t* base, int stride, int n, int64_t tile) {
     // jobs are sorted by tileBase; binary search the last job with base <= tile
     int lo = 0, hi = n - 1;
@@ -623,7 +633,7 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
     __shared__ int sRef[SD * kWgSize];
     __shared__ R sE[SD * kWgSize];
     __shared__ Acc<R, MODE> red[kWgSize / 64];
-    __shared__ int64_t sTile;
+    __shared__ int64_t sNext;
     __shared__ unsigned sPrev;
 
     const int tid = threadIdx.x;
@@ -635,15 +645,20 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
     int ovRef[kMaxStack - SD];
     R ovE[kMaxStack - SD];
 
-    int64_t tile = PERSIST ? 0 : a.tileOffset + blockIdx.x;
-    for (;;) {
-        if (PERSIST) {
-            if (tid == 0) sTile = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
-            __syncthreads();
-            tile = sTile;
-            __syncthreads();
-        }
-        if (tile >= a.totalTiles) break;
+    // Persistent mode pops tiles from a global counter. The pop for the next
+    // tile is made by thread 0 in the same block that bumps the job's
+    // done-counter, so the loop head holds no thread-0-only code: a loop-head
+    // `if (tid == 0)` next to the loop-tail `if (tid == 0)` store lets the
+    // compiler thread the two into a nested loop whose inner back edge skips
+    // the pop for lanes 1..63 of wave 0 (seen in the generated code as a
+    // loop that never ends).
+    int64_t tile = a.tileOffset + blockIdx.x;
+    if (PERSIST) {
+        if (tid == 0) sNext = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
+        __syncthreads();
+        tile = uniform64(sNext);
+    }
+    while (tile < a.totalTiles) {
 
         const int jid = find_job(&a.jobs[0].tileBase, sizeof(Job<R>), a.nJobs, tile);
         const Job<R>& J = a.jobs[jid];
@@ -731,9 +746,12 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
             parts[tile] = tsum;
             __threadfence();
             sPrev = atomicAdd(&a.jobDone[jid], 1u);
+            if (PERSIST) sNext = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
         }
         __syncthreads();
-        if (sPrev == static_cast<unsigned>(J.nTiles - 1)) {
+        const unsigned prev = uniform32(sPrev);
+        const int64_t next = PERSIST ? uniform64(sNext) : a.totalTiles;
+        if (prev == static_cast<unsigned>(J.nTiles - 1)) {
             // last tile of this job: fixed-order reduction of its partials
             __threadfence();
             AccT s = AccT::zero();
@@ -744,7 +762,7 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
                 a.jobOut[2 * jid + 1] = tot.dim();
             }
         }
-        if (!PERSIST) break;
+        tile = next;
     }
 
     if (CNT) {
