@@ -4,17 +4,26 @@
 Workload (one step): trihedral corner reflector + open duct (34,816
 triangles), f = 10 GHz, lambda/10 ray spacing (1.6 m launch grid, 535 x 535
 rays), max 10 bounces, theta 0..90 x phi 0..90 in 1 degree steps (8,281
-angles), all angles in one persistent launch, FP32 traversal with FP64 phase
-and a compensated FP32 complex sum. Mesh, BVH and the job list are resident
-in HBM before the timed region.
+angles), all angles in one persistent launch, mixed precision (FP32
+traversal with error bounds, FP64 ray state and phase, FP64 re-trace of the
+flagged rays). Mesh, BVH and the job list are resident in HBM before the
+timed region; the launch shape is autotuned before it.
 
-Prints one JSON line (metric: angles/s; also ray-bounces/s, the per-angle
-FP32 method on the same GPU, accuracy vs FP64, roofline and a CPU baseline).
+Prints one JSON line: angles/s, ray-bounces/s, the plain per-angle FP32 and
+the batched FP32 / FP64 rates on the same GPU, accuracy vs FP64, the
+roofline (algorithmic bytes from the FP64 traversal counters; HBM traffic
+and VALU activity from rocprofv3 counter passes of the same kernel run in
+child processes) and a CPU baseline (the oracle on the host cores).
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -26,6 +35,8 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 32  # 6 floats of bounds + 2 ints, per node visit of the reference traversal
 TRI_BYTES = 36   # 9 floats, per Moller-Trumbore test
 OUT_BYTES = 16   # one complex per angle
+COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written per launch
+DOMINANT = "trace_kernel"
 
 WORKLOAD = dict(theta_start=0.0, theta_end=90.0, theta_samples=91, phi_start=0.0, phi_end=90.0, phi_samples=91,
                 grid_size=1.6, n_x=535, n_y=535, max_bounces=10, freq=10e9, reflection_const=1.0)
@@ -39,7 +50,9 @@ def parse():
     ap.add_argument("--precision", default="mixed", choices=["fp32", "fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the baseline sample")
-    ap.add_argument("--per-angle-steps", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--tuning", default="", help="JSON tuning to apply instead of autotuning")
     ap.add_argument("--profile-run", action="store_true",
                     help="only the timed batched sweeps (for counter collection under rocprofv3)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -47,16 +60,20 @@ def parse():
     return ap.parse_args()
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def copy_calibration():
     import torch
 
-    n = 1 << 28  # 2^28 floats = 1 GiB
+    n = COPY_BYTES // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+    print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "launches": 3}))
 
 
 def cpu_baseline(verts, faces, kw, target_s):
@@ -83,6 +100,82 @@ def cpu_baseline(verts, faces, kw, target_s):
             "sample": f"{m} of {n} angles (seeded random), FP32 oracle (reference arithmetic), {dt:.1f} s"}
 
 
+def _pmc_pass(counters, extra_args, tmp, name):
+    """One rocprofv3 counter pass over a child bench process; returns
+    {kernel_name: [values per dispatch]} summed over counter instances."""
+    out = os.path.join(tmp, name)
+    cmd = ["rocprofv3", "--pmc", *counters, "--output-format", "csv", "-d", out, "-o", name, "--",
+           sys.executable, os.path.join(ROOT, "bench.py"), *extra_args]
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=180)
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    if r.returncode != 0 and not files:
+        raise RuntimeError(f"rocprofv3 {name} exited {r.returncode}: {r.stdout[-400:]!r}")
+    res = {}
+    for f in files:
+        for row in csv.DictReader(open(f)):
+            key = (row["Kernel_Name"], row["Counter_Name"])
+            res.setdefault(key, {}).setdefault(row["Dispatch_Id"], 0.0)
+            res[key][row["Dispatch_Id"]] += float(row["Counter_Value"])
+    return res
+
+
+def _per_dispatch(res, kernel_sub, counter):
+    vals = [v for (k, c), d in res.items() if kernel_sub in k and c == counter for v in d.values()]
+    if not vals:
+        raise RuntimeError(f"no {counter} rows for {kernel_sub}")
+    return float(np.mean(vals))
+
+
+def pmc_measure(tuning):
+    """HBM bytes per launch of the dominant kernel (FETCH_SIZE and WRITE_SIZE,
+    each scaled by the same counter's reading for a 1 GiB device copy) and
+    its VALU busy / utilisation / occupancy."""
+    if shutil.which("rocprofv3") is None:
+        return {"error": "rocprofv3 not found"}
+    tmp = tempfile.mkdtemp(prefix="sbr_pmc_", dir="/tmp")
+    prof = ["--profile-run", "--steps", "1", "--warmup", "0", "--tuning", json.dumps(tuning)]
+    try:
+        f_b = _per_dispatch(_pmc_pass(["FETCH_SIZE"], prof, tmp, "fetch"), DOMINANT, "FETCH_SIZE")
+        w_b = _per_dispatch(_pmc_pass(["WRITE_SIZE"], prof, tmp, "write"), DOMINANT, "WRITE_SIZE")
+        f_c = _per_dispatch(_pmc_pass(["FETCH_SIZE"], ["--copy-calibration"], tmp, "cfetch"), "copyBuffer",
+                            "FETCH_SIZE")
+        w_c = _per_dispatch(_pmc_pass(["WRITE_SIZE"], ["--copy-calibration"], tmp, "cwrite"), "copyBuffer",
+                            "WRITE_SIZE")
+        v = _pmc_pass(["VALUBusy", "VALUUtilization"], prof, tmp, "valu")
+        o = _pmc_pass(["MeanOccupancyPerCU"], prof, tmp, "occ")
+        # counters are in KiB per dispatch; scale each by copy bytes / copy reading
+        fs = COPY_BYTES / (f_c * 1024.0)
+        ws = COPY_BYTES / (w_c * 1024.0)
+        read = f_b * 1024.0 * fs
+        write = w_b * 1024.0 * ws
+        return {"read_bytes": read, "write_bytes": write, "fetch_scale": fs, "write_scale": ws,
+                "valu_busy_pct": _per_dispatch(v, DOMINANT, "VALUBusy"),
+                "valu_utilization_pct": _per_dispatch(v, DOMINANT, "VALUUtilization"),
+                "mean_occupancy_waves_per_cu": _per_dispatch(o, DOMINANT, "MeanOccupancyPerCU")}
+    except Exception as e:  # counters are reported beside the measurement, never instead of it
+        return {"error": str(e)[:300]}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def timed(plan, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        plan.run()
+        plan.wait()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.run()
+        plan.wait()
+        kernel_ms.append(plan.last_ms())
+    torch.cuda.synchronize()
+    return 1000.0 * (time.perf_counter() - t0) / steps, kernel_ms
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -105,59 +198,52 @@ def main():
     jobs = S.sweep_jobs(params, mesh, prec)
     plan = ctx.plan(jobs, prec, S.SBR_BATCHED)
     n_angles = params.n_angles
+    if args.tuning:
+        t = json.loads(args.tuning)
+        plan.tune(t["wg_per_cu"], t["lds_stack"], t["tile_rays"], t["use_graph"], t.get("ray_pool", 0))
+        tuning = t
+    elif not args.no_autotune:
+        tuning = plan.autotune(sample_jobs=0, repeats=1)
+    else:
+        tuning = {"wg_per_cu": 0, "lds_stack": 0, "tile_rays": 0, "use_graph": 0, "ray_pool": 0}
 
-    for _ in range(args.warmup):
-        plan.run()
-        plan.wait()
-    torch.cuda.synchronize()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.run()
-        plan.wait()
-        kernel_ms.append(plan.last_ms())
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1000.0 * wall / args.steps
+    log(f"tuning {tuning}; timing {args.steps} steps")
+    ms_per_step, kernel_ms = timed(plan, args.steps, args.warmup)
+    log(f"{ms_per_step:.3f} ms/step")
     z, sigma, dbsm = plan.results()
     if args.profile_run:
         print(json.dumps({"profile_run": True, "ms_per_step": ms_per_step, "kernel_ms": kernel_ms}))
         return
-
-    # work counters (separate, untimed runs): the reference traversal's node
-    # visits and triangle tests (FP32 plain path) give the algorithmic bytes;
-    # the measured plan's own run gives its re-trace count
-    jobs32 = S.sweep_jobs(params, mesh, S.SBR_FP32)
-    ref32 = ctx.plan(jobs32, S.SBR_FP32, S.SBR_BATCHED)
-    cnt = ref32.count()
     own = plan.count()
-    cnt["retraced"] = own["retraced"]
+
+    # FP64 batched plan: accuracy reference and the reference traversal's
+    # node visits / triangle tests (bit-exact with the FP64 oracle), which
+    # give the algorithmic bytes
+    log("fp64 batched plan")
+    jobs64 = S.sweep_jobs(params, mesh, S.SBR_FP64)
+    p64 = ctx.plan(jobs64, S.SBR_FP64, S.SBR_BATCHED)
+    t64 = p64.autotune(sample_jobs=0, repeats=1) if not args.no_autotune else None
+    fp64_ms, _ = timed(p64, 2, 1)
+    z64, s64, db64 = p64.results()
+    cnt = p64.count()
+    derr = np.abs(dbsm - db64)
+    strong = db64 > db64.max() - 30.0
     alg_bytes = cnt["node_visits"] * NODE_BYTES + cnt["tri_tests"] * TRI_BYTES + OUT_BYTES * n_angles
     k_ms = float(np.mean(kernel_ms))
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
 
-    # accuracy vs the FP64 path on the same GPU
-    jobs64 = S.sweep_jobs(params, mesh, S.SBR_FP64)
-    p64 = ctx.plan(jobs64, S.SBR_FP64, S.SBR_BATCHED)
-    p64.run()
-    p64.wait()
-    z64, s64, db64 = p64.results()
-    fp64_ms = p64.last_ms()
-    derr = np.abs(dbsm - db64)
-    peak_db = db64.max()
-    strong = db64 > peak_db - 30.0
-
-    # plain per-angle FP32 kernel on the same GPU (one launch per angle)
+    # FP32 batched and the plain per-angle FP32 method (one launch per angle,
+    # its own autotuned launch shape), same GPU
+    log("fp32 batched and per-angle plans")
+    jobs32 = S.sweep_jobs(params, mesh, S.SBR_FP32)
+    b32 = ctx.plan(jobs32, S.SBR_FP32, S.SBR_BATCHED)
+    if not args.no_autotune:
+        b32.autotune(sample_jobs=0, repeats=1)
+    fp32_ms, _ = timed(b32, 2, 1)
     pa = ctx.plan(jobs32, S.SBR_FP32, S.SBR_PER_ANGLE)
-    pa.run()
-    pa.wait()
-    pa_t = []
-    for _ in range(max(1, args.per_angle_steps)):
-        t1 = time.perf_counter()
-        pa.run()
-        pa.wait()
-        pa_t.append(time.perf_counter() - t1)
-    per_angle_rate = n_angles / float(np.mean(pa_t))
+    t_pa = pa.autotune(sample_jobs=512, repeats=1) if not args.no_autotune else None
+    pa_ms, _ = timed(pa, 1, 1)
+    per_angle_rate = n_angles / (pa_ms / 1000.0)
 
     value = n_angles / (ms_per_step / 1000.0)
     out = {
@@ -175,22 +261,40 @@ def main():
         "data": "synthetic (procedural trihedral + duct mesh)",
         "config": {"workload": "trihedral+duct multi-bounce sweep: 34,816 triangles, 10 GHz, 535x535 rays "
                                "(lambda/10 over 1.6 m), max 10 bounces, theta 0..90 x phi 0..90 (8,281 angles), "
-                               "one persistent launch", "method": "batched", "precision": args.precision},
+                               "one persistent launch", "method": "batched", "precision": args.precision,
+                   "tuning": tuning},
         "ray_bounces_per_s": round(cnt["ray_bounces"] / (ms_per_step / 1000.0), 1),
-        "rays_traced_per_s": round(cnt["rays"] / (ms_per_step / 1000.0), 1),
+        "bvh_queries_per_s": round(cnt["queries"] / (ms_per_step / 1000.0), 1),
         "per_angle_fp32_angles_per_s": round(per_angle_rate, 2),
+        "per_angle_fp32_tuning": t_pa,
         "batched_over_per_angle": round(value / per_angle_rate, 3),
-        "fp64_angles_per_s": round(n_angles / (fp64_ms / 1000.0), 2),
+        "batched_fp32_angles_per_s": round(n_angles / (fp32_ms / 1000.0), 2),
+        "batched_fp64_angles_per_s": round(n_angles / (fp64_ms / 1000.0), 2),
         "accuracy_vs_fp64_db": {"max": float(derr.max()), "p99": float(np.percentile(derr, 99)),
                                 "median": float(np.median(derr)),
                                 "max_within_30dB_of_peak": float(derr[strong].max()),
                                 "frac_angles_le_0.1dB": float(np.mean(derr <= 0.1))},
-        "counters": cnt,
+        "counters_fp64_traversal": cnt,
+        "retraced_rays": own["retraced"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel_ms": round(k_ms, 3), "algorithmic_bytes": int(alg_bytes)},
     }
+    if not args.no_pmc:
+        log("rocprofv3 counter passes")
+        pm = pmc_measure(tuning)
+        if "error" not in pm:
+            out["roofline"]["traffic"] = int(pm["read_bytes"] + pm["write_bytes"])
+            out["roofline"]["traffic_read"] = int(pm["read_bytes"])
+            out["roofline"]["traffic_write"] = int(pm["write_bytes"])
+            out["roofline"]["hbm_gbs"] = round((pm["read_bytes"] + pm["write_bytes"]) / (k_ms * 1e-3) / 1e9, 2)
+            out["valu"] = {k: round(pm[k], 3) for k in ("valu_busy_pct", "valu_utilization_pct",
+                                                       "mean_occupancy_waves_per_cu")}
+            out["pmc_scale"] = {"fetch": round(pm["fetch_scale"], 4), "write": round(pm["write_scale"], 4)}
+        else:
+            out["pmc_error"] = pm["error"]
     if not args.no_cpu_baseline:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(verts, faces, WORKLOAD, args.cpu_seconds)
     print(json.dumps(out))
 

@@ -86,6 +86,8 @@ typedef struct {
     int lds_stack;   /* traversal stack entries kept in LDS per ray (8, 16, 32) */
     int tile_rays;   /* rays per tile side: 16, 32 or 64 */
     int use_graph;   /* 1: capture the run's launches in a hipGraph once and replay it */
+    int ray_pool;    /* 1: lanes refill from the tile's ray pool after each ray;
+                        0 or 2: one ray per lane per 16 x 16 pass (default) */
 } sbr_tuning;
 
 /* Mesh / BVH statistics. */
@@ -180,6 +182,18 @@ int sbr_plan_results(sbr_plan* plan, double* reim, double* sigma, double* dbsm);
 /* Run once with work counters enabled (slower; not for timing). */
 int sbr_plan_count(sbr_plan* plan, sbr_counters* out);
 int sbr_plan_destroy(sbr_plan* plan);
+
+/* Autotuner (no counterpart in the reference, whose launch shape is fixed at
+ * 16 x 16 threads per block, src/app/sweep.cu:385-398). Times the plan's
+ * runs over ray pool {on, off} x LDS stack {8, 16, 32} x tile side
+ * {16, 32, 64} x persistent workgroups per CU {occupancy, 2} (batched), or
+ * x tile side {16, 32} x hipGraph replay {0, 1} (per-angle), keeps the
+ * fastest and applies it to the plan. sample_jobs > 0
+ * tunes on an evenly strided subset of that many jobs (for large sweeps);
+ * repeats >= 1 timed runs per candidate (the minimum wall time counts).
+ * Results do not depend on the tuning: partial sums are kept per 16 x 16 ray
+ * pass and reduced in a fixed order. best / best_ms / n_tried may be NULL. */
+int sbr_plan_autotune(sbr_plan* plan, int sample_jobs, int repeats, sbr_tuning* best, double* best_ms, int* n_tried);
 
 /* One-call sweep: plan_create + run + wait + results + destroy. */
 int sbr_sweep(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precision, int method, double* reim,

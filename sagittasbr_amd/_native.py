@@ -92,7 +92,8 @@ class Job(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("wg_per_cu", C.c_int), ("lds_stack", C.c_int), ("tile_rays", C.c_int), ("use_graph", C.c_int)]
+    _fields_ = [("wg_per_cu", C.c_int), ("lds_stack", C.c_int), ("tile_rays", C.c_int), ("use_graph", C.c_int),
+                ("ray_pool", C.c_int)]
 
 
 class MeshInfo(C.Structure):
@@ -163,6 +164,8 @@ def lib():
         "sbr_sweep_jobs": (C.c_int, [C.POINTER(SweepParams), C.c_int, C.c_int, _P, C.c_int]),
         "sbr_plan_create": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
         "sbr_plan_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+        "sbr_plan_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int)]),
         "sbr_plan_run": (C.c_int, [_P]),
         "sbr_plan_wait": (C.c_int, [_P]),
         "sbr_plan_last_ms": (C.c_int, [_P, f64p]),
@@ -267,9 +270,18 @@ class Plan:
         self.h = _P()
         _check(lib().sbr_plan_create(ctx.h, C.cast(jobs, _P), self.n, precision, method, C.byref(self.h)), ctx.h)
 
-    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0):
-        _check(lib().sbr_plan_set_tuning(self.h, C.byref(Tuning(wg_per_cu, lds_stack, tile_rays, use_graph))),
+    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0, ray_pool=0):
+        t = Tuning(wg_per_cu, lds_stack, tile_rays, use_graph, ray_pool)
+        _check(lib().sbr_plan_set_tuning(self.h, C.byref(t)), self.ctx.h)
+
+    def autotune(self, sample_jobs=0, repeats=1):
+        """Time the plan over the tuning space, apply and return the fastest
+        (results are independent of the tuning)."""
+        t, ms, n = Tuning(), C.c_double(), C.c_int()
+        _check(lib().sbr_plan_autotune(self.h, sample_jobs, repeats, C.byref(t), C.byref(ms), C.byref(n)),
                self.ctx.h)
+        return {"wg_per_cu": t.wg_per_cu, "lds_stack": t.lds_stack, "tile_rays": t.tile_rays,
+                "use_graph": t.use_graph, "ray_pool": t.ray_pool, "best_ms": ms.value, "candidates": n.value}
 
     def run(self):
         _check(lib().sbr_plan_run(self.h), self.ctx.h)

@@ -24,14 +24,14 @@
 
 namespace sbr {
 template <typename R>
-hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, int grid,
+hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool, int grid,
                         hipStream_t s);
 hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s);
 hipError_t launch_retrace(const TraceArgs<float>& a, bool rec, bool cnt, int grid, hipStream_t s);
 hipError_t launch_finalize(const Job<double>* jobs64, int n, const unsigned long long* fix, double* out,
                            hipStream_t s);
 size_t partial_bytes(bool fp64, int mode);
-template <typename R> int trace_occupancy(int mode, int sd);
+template <typename R> int trace_occupancy(int mode, int sd, bool pool);
 }  // namespace sbr
 
 using namespace sbr;
@@ -115,7 +115,7 @@ struct sbr_plan {
     std::vector<int> nTiles;
     int64_t totalTiles = 0;
     int maxDepth = 0;
-    sbr_tuning tune{0, 0, 0, 0};
+    sbr_tuning tune{0, 0, 0, 0, 0};
     int tileSide = 32;
     DevBuf dJobs, dPartials, dJobOut, dJobDone, dCounter, dCounters;
     // mixed mode
@@ -297,7 +297,7 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
     pl->tileBase.assign(pl->nJobs, 0);
     pl->nTiles.assign(pl->nJobs, 0);
     int64_t base = 0;
-    int64_t rays = 0;
+    int64_t rays = 0, passes = 0;
     int maxDepth = 0;
     const int ts = pl->tileSide;
     for (int i = 0; i < pl->nJobs; ++i) {
@@ -367,13 +367,20 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         J.tilesY = any ? (j1 - j0 + ts) / ts : 0;
         J.nTiles = J.tilesX * J.tilesY;
         J.tileBase = base;
+        J.passX = any ? (i1 - i0 + 16) / 16 : 0;
+        J.passY = any ? (j1 - j0 + 16) / 16 : 0;
+        J.passBase = passes;
         pl->tileBase[i] = base;
         pl->nTiles[i] = J.nTiles;
         base += J.nTiles;
+        passes += int64_t(J.passX) * J.passY;
         rays += int64_t(J.nTiles) * ts * ts;
         if (mixed) {
             jobs64[i].tileBase = J.tileBase;
             jobs64[i].nTiles = J.nTiles;
+            jobs64[i].passX = J.passX;
+            jobs64[i].passY = J.passY;
+            jobs64[i].passBase = J.passBase;
         }
         jobs[i] = J;
     }
@@ -384,7 +391,7 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
     const int mode = mixed ? 1 : 0;
     HIP_TRY(&ctx->err, pl->dJobs.ensure(sizeof(Job<R>) * std::max(pl->nJobs, 1)));
     HIP_TRY(&ctx->err, hipMemcpy(pl->dJobs.p, jobs.data(), sizeof(Job<R>) * pl->nJobs, hipMemcpyHostToDevice));
-    HIP_TRY(&ctx->err, pl->dPartials.ensure(partial_bytes(sizeof(R) == 8, mode) * std::max<int64_t>(base, 1)));
+    HIP_TRY(&ctx->err, pl->dPartials.ensure(partial_bytes(sizeof(R) == 8, mode) * std::max<int64_t>(passes, 1)));
     HIP_TRY(&ctx->err, pl->dJobOut.ensure(sizeof(double) * 2 * std::max(pl->nJobs, 1)));
     HIP_TRY(&ctx->err, pl->dJobDone.ensure(sizeof(unsigned) * std::max(pl->nJobs, 1)));
     HIP_TRY(&ctx->err, pl->dCounter.ensure(sizeof(unsigned long long)));
@@ -406,6 +413,12 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
 
 int default_sd(int /*depth*/) { return 8; }
 
+// One angle launched on its own fills the GPU only with small tiles (a
+// 535 x 535 grid gives ~1,100 tiles of 16 x 16 but ~280 of 32 x 32); the
+// persistent sweep prefers the larger tile, which halves barrier and
+// reduction work per ray.
+int default_tile(int method) { return method == SBR_PER_ANGLE ? 16 : 32; }
+
 void drop_graph(sbr_plan* pl) {
     if (pl->graphExec) (void)hipGraphExecDestroy(pl->graphExec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
@@ -420,6 +433,10 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     sbr_ctx* ctx = pl->ctx;
     const int mode = pl->precision == SBR_MIXED ? 1 : 0;
     const int sd = pl->tune.lds_stack > 0 ? pl->tune.lds_stack : default_sd(pl->maxDepth);
+    // measured on the trihedral+duct sweep: refilling lanes mixes bounce
+    // depths within a wave and costs more in traversal coherence than it
+    // gains in lane use, so the pool is opt-in (the autotuner tries both)
+    const bool pool = pl->tune.ray_pool == 1 && !record;
     TraceArgs<R> a{};
     a.jobs = static_cast<const Job<R>*>(pl->dJobs.p);
     a.meshes = static_cast<const Mesh<R>*>(sizeof(R) == 4 ? ctx->meshTable32.p : ctx->meshTable64.p);
@@ -459,17 +476,17 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
     if (pl->totalTiles > 0) {
         if (pl->method == SBR_BATCHED && !record) {
-            int occ = trace_occupancy<R>(mode, sd);
+            int occ = trace_occupancy<R>(mode, sd, pool);
             int per = pl->tune.wg_per_cu > 0 ? pl->tune.wg_per_cu : std::max(occ, 1);
             int64_t grid = int64_t(ctx->cuCount) * per;
             grid = std::min<int64_t>(grid, pl->totalTiles);
             a.tileOffset = 0;
-            HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, false, count, true, int(grid), s));
+            HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, false, count, true, pool, int(grid), s));
         } else {
             for (int i = 0; i < pl->nJobs; ++i) {
                 if (pl->nTiles[i] == 0) continue;
                 a.tileOffset = pl->tileBase[i];
-                HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, record, count, false, pl->nTiles[i], s));
+                HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, record, count, false, pool, pl->nTiles[i], s));
             }
         }
     }
@@ -549,6 +566,7 @@ int create_plan(sbr_ctx* ctx, const sbr_job* jobs, int n, int precision, int met
     pl->precision = precision;
     pl->method = method;
     pl->nJobs = n;
+    pl->tileSide = default_tile(method);
     pl->jobs.assign(jobs, jobs + n);
     HIP_TRY(&ctx->err, hipEventCreate(&pl->ev0));
     HIP_TRY(&ctx->err, hipEventCreate(&pl->ev1));
@@ -866,11 +884,12 @@ int sbr_plan_create(sbr_ctx* ctx, const sbr_job* jobs, int n_jobs, int precision
 int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
     if (!pl || !t) return fail(nullptr, "null argument");
     if (t->lds_stack < 0 || t->wg_per_cu < 0) return fail(&pl->ctx->err, "negative tuning value");
+    if (t->ray_pool < 0 || t->ray_pool > 2) return fail(&pl->ctx->err, "ray_pool must be 0, 1 or 2");
     if (t->tile_rays != 0 && t->tile_rays != 16 && t->tile_rays != 32 && t->tile_rays != 64)
         return fail(&pl->ctx->err, "tile_rays must be 16, 32 or 64");
     drop_graph(pl);
     pl->tune = *t;
-    int ts = t->tile_rays ? t->tile_rays : 32;
+    int ts = t->tile_rays ? t->tile_rays : default_tile(pl->method);
     if (ts != pl->tileSide) {
         pl->tileSide = ts;
         if (dispatch_prepare(pl, false)) return 1;
@@ -939,6 +958,60 @@ int sbr_plan_count(sbr_plan* pl, sbr_counters* out) {
     out->ray_bounces = c[3] + c[8];
     out->queries = c[4] + c[9];
     out->retraced = c[5];
+    return 0;
+}
+
+int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* best, double* best_ms, int* n_tried) {
+    if (!pl) return fail(nullptr, "null plan");
+    sbr_ctx* ctx = pl->ctx;
+    if (repeats < 1) return fail(&ctx->err, "repeats must be >= 1");
+    HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
+    // probe plan: the plan itself, or an evenly strided job subset
+    sbr_plan* probe = pl;
+    sbr_plan* owned = nullptr;
+    if (sample_jobs > 0 && sample_jobs < pl->nJobs) {
+        std::vector<sbr_job> sub;
+        sub.reserve(sample_jobs);
+        for (int i = 0; i < sample_jobs; ++i) sub.push_back(pl->jobs[int64_t(i) * pl->nJobs / sample_jobs]);
+        if (create_plan(ctx, sub.data(), int(sub.size()), pl->precision, pl->method, false, &owned)) return 1;
+        probe = owned;
+    }
+    std::vector<sbr_tuning> cands;
+    for (int pool : {1, 2})
+        for (int sd : {8, 16, 32}) {
+            if (pl->method == SBR_BATCHED) {
+                for (int ts : {16, 32, 64})
+                    for (int wg : {0, 2}) cands.push_back(sbr_tuning{wg, sd, ts, 0, pool});
+            } else {
+                for (int ts : {16, 32})
+                    for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool});
+            }
+        }
+    double bestT = 1e300;
+    sbr_tuning bestC = cands[0];
+    int rc = 0;
+    for (const sbr_tuning& c : cands) {
+        if ((rc = sbr_plan_set_tuning(probe, &c))) break;
+        // one untimed run (graph capture, caches), then the timed ones
+        if ((rc = sbr_plan_run(probe)) || (rc = sbr_plan_wait(probe))) break;
+        double t = 1e300;
+        for (int r = 0; r < repeats; ++r) {
+            auto t0 = std::chrono::steady_clock::now();
+            if ((rc = sbr_plan_run(probe)) || (rc = sbr_plan_wait(probe))) break;
+            t = std::min(t, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        if (rc) break;
+        if (t < bestT) {
+            bestT = t;
+            bestC = c;
+        }
+    }
+    if (owned) sbr_plan_destroy(owned);
+    if (rc) return rc;
+    if (sbr_plan_set_tuning(pl, &bestC)) return 1;
+    if (best) *best = bestC;
+    if (best_ms) *best_ms = bestT;
+    if (n_tried) *n_tried = int(cands.size());
     return 0;
 }
 

@@ -25,6 +25,11 @@ template <typename R> struct alignas(16) Job {
     int i0, j0, tilesX, tilesY;      // culled ray rectangle in tiles
     int64_t tileBase;                // first global tile of this job
     int nTiles;
+    // Partial sums are kept per 16x16 ray pass on a grid fixed by the culled
+    // rectangle alone (passX x passY), so the summation order -- and sigma to
+    // the last bit -- does not depend on the tile size or on the method.
+    int passX, passY;
+    int64_t passBase;                // first partial of this job
 };
 
 template <typename R> struct alignas(16) Mesh {
@@ -49,7 +54,7 @@ template <typename R> struct TraceArgs {
     int tileSide;            // rays per tile side (8, 16 or 32)
     int64_t totalTiles;
     int64_t tileOffset;      // first tile of a non-persistent launch
-    void* partials;          // one accumulator per tile
+    void* partials;          // one accumulator per 16x16 ray pass
     double* jobOut;          // 2 doubles per job
     unsigned* jobDone;       // tiles finished per job
     unsigned long long* tileCounter;  // persistent queue head

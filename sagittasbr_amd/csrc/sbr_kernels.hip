@@ -101,6 +101,17 @@ __device__ __forceinline__ bool slab(R lx, R ly, R lz, R hx, R hy, R hz, V3<R> o
     return entry <= tf;
 }
 
+// BVH and triangle records are read through global-address-space pointers:
+// the mesh table holds them as plain pointers, and without the cast every
+// fetch is a FLAT access (which also waits on the LDS counter).
+template <typename T> using gptr = const __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> as_global(const void* p) {
+    return (gptr<T>)(p);
+}
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef double vd2 __attribute__((ext_vector_type(2)));
+typedef int vi4 __attribute__((ext_vector_type(4)));
+
 template <typename R> struct TriRec {
     V3<R> v0, e1, e2;
     int face;
@@ -110,24 +121,24 @@ template <typename R> struct TriRec {
 // FP32 record: v0, v1, v2, face, |e1|+|e2|, |e1||e2|; the edges are formed
 // here exactly as rtKernels.cu:13-14 forms them.
 __device__ __forceinline__ TriRec<float> load_tri(const float* tris, int i) {
-    const float4* p = reinterpret_cast<const float4*>(tris) + 3 * i;
-    float4 a = p[0], b = p[1], c = p[2];
+    gptr<vf4> p = as_global<vf4>(tris) + 3 * i;
+    vf4 a = p[0], b = p[1], c = p[2];
     V3<float> v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
     return {v0, sub(v1, v0), sub(v2, v0), __float_as_int(c.y), c.z, c.w};
 }
 // Exact FP64 vertices of an FP32 record (for the mixed mode's recompute).
 __device__ __forceinline__ void load_tri_verts64(const float* tris, int i, V3<double>& v0, V3<double>& e1,
                                                  V3<double>& e2, int& face) {
-    const float4* p = reinterpret_cast<const float4*>(tris) + 3 * i;
-    float4 a = p[0], b = p[1], c = p[2];
+    gptr<vf4> p = as_global<vf4>(tris) + 3 * i;
+    vf4 a = p[0], b = p[1], c = p[2];
     v0 = {double(a.x), double(a.y), double(a.z)};
     e1 = {double(a.w) - v0.x, double(b.x) - v0.y, double(b.y) - v0.z};
     e2 = {double(b.z) - v0.x, double(b.w) - v0.y, double(c.x) - v0.z};
     face = __float_as_int(c.y);
 }
 __device__ __forceinline__ TriRec<double> load_tri(const double* tris, int i) {
-    const double2* p = reinterpret_cast<const double2*>(tris) + 5 * i;
-    double2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    gptr<vd2> p = as_global<vd2>(tris) + 5 * i;
+    vd2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
     int face = static_cast<int>(__double_as_longlong(e.y) & 0xffffffffll);
     return {{a.x, a.y, b.x}, {b.y, c.x, c.y}, {d.x, d.y, e.x}, face, 0.f, 0.f};
 }
@@ -137,21 +148,21 @@ template <typename R> struct NodeRec {
     int l, r;
 };
 __device__ __forceinline__ NodeRec<float> load_node(const float* nodes, int i) {
-    const float4* p = reinterpret_cast<const float4*>(nodes) + 4 * i;
-    float4 a = p[0], b = p[1], c = p[2];
-    int4 d = reinterpret_cast<const int4*>(p)[3];
+    gptr<vf4> p = as_global<vf4>(nodes) + 4 * i;
+    vf4 a = p[0], b = p[1], c = p[2];
+    vi4 d = as_global<vi4>(nodes)[4 * i + 3];
     return {{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w}, d.x, d.y};
 }
 __device__ __forceinline__ NodeRec<double> load_node(const double* nodes, int i) {
-    const double2* p = reinterpret_cast<const double2*>(nodes) + 8 * i;
+    gptr<vd2> p = as_global<vd2>(nodes) + 8 * i;
     NodeRec<double> n;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        double2 v = p[k];
+        vd2 v = p[k];
         n.b[2 * k] = v.x;
         n.b[2 * k + 1] = v.y;
     }
-    int4 d = reinterpret_cast<const int4*>(p + 6)[0];
+    vi4 d = as_global<vi4>(nodes)[8 * i + 6];
     n.l = d.x;
     n.r = d.y;
     return n;
@@ -176,6 +187,30 @@ __device__ __forceinline__ bool mt_test(const TriRec<R>& tr, V3<R> o, V3<R> d, R
     return true;
 }
 
+// Private overflow entry of the traversal stack: (slab entry, node ref)
+// packed in one 64-bit (FP32) or 128-bit (FP64) word, so an overflow pop is
+// one scratch load whose shape differs from the LDS pop (the compiler would
+// otherwise merge the two into a FLAT access).
+template <typename R> struct Ov;
+template <> struct Ov<float> {
+    using T = unsigned long long;
+    __device__ static T pack(float e, int r) {
+        return (static_cast<T>(static_cast<unsigned>(r)) << 32) | static_cast<T>(__float_as_uint(e));
+    }
+    __device__ static void unpack(T v, float& e, int& r) {
+        e = __uint_as_float(static_cast<unsigned>(v));
+        r = static_cast<int>(static_cast<unsigned>(v >> 32));
+    }
+};
+template <> struct Ov<double> {
+    using T = vd2;
+    __device__ static T pack(double e, int r) { return T{e, __longlong_as_double(static_cast<long long>(r))}; }
+    __device__ static void unpack(T v, double& e, int& r) {
+        e = v.x;
+        r = static_cast<int>(__double_as_longlong(v.y));
+    }
+};
+
 struct Work {
     unsigned visits, tests;
 };
@@ -187,7 +222,7 @@ struct Work {
 // the host guarantees tree depth <= kMaxDepth so sp < kMaxStack.
 template <typename R, int SD, bool CNT>
 __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V3<R> inv, R& tHit, int* sRef, R* sE,
-                                           int* ovRef, R* ovE, int tid, Work& w) {
+                                           typename Ov<R>::T* ov, int tid, Work& w) {
     const R* nodes = static_cast<const R*>(m.nodes);
     const R* tris = static_cast<const R*>(m.tris);
     R closest = R(1e20);
@@ -210,8 +245,7 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
                         sRef[sp * kWgSize + tid] = n.l;
                         sE[sp * kWgSize + tid] = eL;
                     } else {
-                        ovRef[sp - SD] = n.l;
-                        ovE[sp - SD] = eL;
+                        ov[sp - SD] = Ov<R>::pack(eL, n.l);
                     }
                     ++sp;
                 }
@@ -245,8 +279,7 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
                 ee = sE[sp * kWgSize + tid];
                 rr = sRef[sp * kWgSize + tid];
             } else {
-                ee = ovE[sp - SD];
-                rr = ovRef[sp - SD];
+                Ov<R>::unpack(ov[sp - SD], ee, rr);
             }
             if (ee <= closest) {
                 cur = rr;
@@ -265,7 +298,7 @@ constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
 
 template <int SD, bool CNT>
 __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> d, V3<float> inv,
-                                                 float& tHit, int* sRef, float* sE, int* ovRef, float* ovE, int tid,
+                                                 float& tHit, int* sRef, float* sE, Ov<float>::T* ov, int tid,
                                                  Work& w, float ex, float ed, float K, bool& amb) {
     const float* nodes = static_cast<const float*>(m.nodes);
     const float* tris = static_cast<const float*>(m.tris);
@@ -310,8 +343,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
                         sRef[sp * kWgSize + tid] = n.l;
                         sE[sp * kWgSize + tid] = eL;
                     } else {
-                        ovRef[sp - SD] = n.l;
-                        ovE[sp - SD] = eL;
+                        ov[sp - SD] = Ov<float>::pack(eL, n.l);
                     }
                     ++sp;
                 }
@@ -372,8 +404,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
                 ee = sE[sp * kWgSize + tid];
                 rr = sRef[sp * kWgSize + tid];
             } else {
-                ee = ovE[sp - SD];
-                rr = ovRef[sp - SD];
+                Ov<float>::unpack(ov[sp - SD], ee, rr);
             }
             if (ee <= closest + tErr) {
                 cur = rr;
@@ -487,42 +518,71 @@ template <typename R> struct PathOut {
     int count;
     R L;
     double Ld;
-    V3<R> n, d;
+    R cosT;  // n_last . (-d_last), poKernels.cu:89-94, formed at the last hit
 };
 
-// Reference ray (rtKernels.cu:113-158): generation, bounces, path length.
-// SD == 0 keeps the whole traversal stack in private memory.
-template <typename R, int SD, bool CNT, bool REC>
-__device__ __forceinline__ PathOut<R> trace_path(const Job<R>& J, const Mesh<R>& M, int i, int j, int* sRef, R* sE,
-                                                 int* ovRef, R* ovE, int tid, int32_t* ids, Work& w,
-                                                 unsigned long long& queries) {
+// Reference ray (rtKernels.cu:113-158) as a state machine: generation,
+// then one closest-hit query plus bounce update per step, so a lane can be
+// refilled with a new ray as soon as its current one ends.
+template <typename R> struct RayState {
+    V3<R> o, d;
+    PathOut<R> P;
+    int k;
+};
+
+template <typename R> __device__ __forceinline__ V3<R> ray_origin(const Job<R>& J, int i, int j) {
     R u = (R(i) + R(0.5)) / R(J.nx);
     R v = (R(j) + R(0.5)) / R(J.ny);
     V3<R> llc{J.llc[0], J.llc[1], J.llc[2]};
     V3<R> U{J.uv[0], J.uv[1], J.uv[2]}, Vv{J.vv[0], J.vv[1], J.vv[2]};
-    V3<R> o = axpy(v, Vv, axpy(u, U, llc));
-    V3<R> d{J.dir[0], J.dir[1], J.dir[2]};
-    PathOut<R> out{0, R(0), 0.0, {0, 0, 0}, {0, 0, 0}};
-    for (int k = 0; k < J.maxB; ++k) {
-        V3<R> inv{R(1.0) / d.x, R(1.0) / d.y, R(1.0) / d.z};
-        R t;
-        int h = closest_hit<R, SD, CNT>(M, o, d, inv, t, sRef, sE, ovRef, ovE, tid, w);
-        if (CNT) queries++;
-        if (h < 0) break;
-        TriRec<R> tr = load_tri(static_cast<const R*>(M.tris), h);
-        V3<R> n = unit3(cross3(tr.e1, tr.e2));
-        V3<R> p = axpy(t, d, o);
-        out.n = n;
-        out.d = d;
-        out.count++;
-        out.L += t;
-        if (REC) ids[k] = tr.face;
-        R s = R(2.0) * dot3(d, n);
-        d = unit3(axpy(-s, n, d));
-        o = axpy(R(1e-4), n, p);
-    }
-    out.Ld = double(out.L);
-    return out;
+    return axpy(v, Vv, axpy(u, U, llc));
+}
+
+template <typename R> __device__ __forceinline__ void ray_init(const Job<R>& J, int i, int j, RayState<R>& st) {
+    st.o = ray_origin(J, i, j);
+    st.d = V3<R>{J.dir[0], J.dir[1], J.dir[2]};
+    st.P = PathOut<R>{0, R(0), 0.0, R(0)};
+    st.k = 0;
+}
+
+// One query of bounce st.k; returns true when the ray has ended (miss or
+// max_bounces reached). Requires st.k < J.maxB.
+template <typename R, int SD, bool CNT, bool REC>
+__device__ __forceinline__ bool ray_step(const Job<R>& J, const Mesh<R>& M, RayState<R>& st, int* sRef, R* sE,
+                                         typename Ov<R>::T* ov, int tid, int32_t* ids, Work& w,
+                                         unsigned long long& queries) {
+    V3<R> inv{R(1.0) / st.d.x, R(1.0) / st.d.y, R(1.0) / st.d.z};
+    R t;
+    int h = closest_hit<R, SD, CNT>(M, st.o, st.d, inv, t, sRef, sE, ov, tid, w);
+    if (CNT) queries++;
+    if (h < 0) return true;
+    TriRec<R> tr = load_tri(static_cast<const R*>(M.tris), h);
+    V3<R> n = unit3(cross3(tr.e1, tr.e2));
+    V3<R> p = axpy(t, st.d, st.o);
+    st.P.cosT = dot3(n, V3<R>{-st.d.x, -st.d.y, -st.d.z});
+    st.P.count++;
+    st.P.L += t;
+    if (REC) ids[st.k] = tr.face;
+    R s = R(2.0) * dot3(st.d, n);
+    st.d = unit3(axpy(-s, n, st.d));
+    st.o = axpy(R(1e-4), n, p);
+    st.k++;
+    return st.k >= J.maxB;
+}
+
+// Whole path of one ray in one call. SD == 0 keeps the whole traversal
+// stack in private memory.
+template <typename R, int SD, bool CNT, bool REC>
+__device__ __forceinline__ PathOut<R> trace_path(const Job<R>& J, const Mesh<R>& M, int i, int j, int* sRef, R* sE,
+                                                 typename Ov<R>::T* ov, int tid, int32_t* ids, Work& w,
+                                                 unsigned long long& queries) {
+    RayState<R> st;
+    ray_init(J, i, j, st);
+    if (J.maxB > 0)
+        while (!ray_step<R, SD, CNT, REC>(J, M, st, sRef, sE, ov, tid, ids, w, queries)) {
+        }
+    st.P.Ld = double(st.P.L);
+    return st.P;
 }
 
 // PO term (poKernels.cu:75-110) from a finished path; the phase uses the
@@ -534,8 +594,7 @@ __device__ __forceinline__ void po_term(const Job<R>& J, const PathOut<R>& P, bo
     if (P.count <= 0) return;
     R refl = R(1.0);
     for (int b = 0; b < P.count; ++b) refl *= J.refl;
-    V3<R> nd{-P.d.x, -P.d.y, -P.d.z};
-    R cosT = dot3(P.n, nd);
+    const R cosT = P.cosT;
     if (!(cosT > R(0))) return;
     R ph;
     if (fp64Phase) ph = static_cast<R>(fmodr((2.0 * J.kd) * P.Ld, J.twoPid));
@@ -552,58 +611,71 @@ __device__ __forceinline__ void po_term(const Job<R>& J, const PathOut<R>& P, bo
 // the FP64 reference ray itself; each closest-hit query runs on an FP32 copy
 // of it (the first one from a launch plane moved next to the body) with
 // error-bounded decisions, and the chosen triangle's distance is then
-// recomputed in FP64. Returns false when a query was ambiguous.
+// recomputed in FP64.
+__device__ __forceinline__ void ray_init_mixed(const Job<double>& D, int i, int j, RayState<double>& st) {
+    st.o = ray_origin(D, i, j);
+    st.d = V3<double>{D.dir[0], D.dir[1], D.dir[2]};
+    st.P = PathOut<double>{0, 0.0, 0.0, 0.0};
+    st.k = 0;
+}
+
+// One query of bounce st.k: 0 = continue, 1 = ray ended, 2 = the query was
+// ambiguous in FP32 (the ray must be re-traced in FP64).
+template <int SD, bool CNT, bool REC>
+__device__ __forceinline__ int ray_step_mixed(const Job<float>& J, const Mesh<float>& M, RayState<double>& st,
+                                              int* sRef, float* sE, Ov<float>::T* ov, int tid, int32_t* ids,
+                                              Work& w, unsigned long long& queries, float K) {
+    const V3<double> o = st.o, d = st.d;
+    // FP32 copy of the ray; the launch-plane move only shifts the origin
+    // along the ray, which changes no decision
+    const double shift = st.k == 0 ? J.L0 : 0.0;
+    V3<float> of{float(o.x + shift * d.x), float(o.y + shift * d.y), float(o.z + shift * d.z)};
+    V3<float> df{float(d.x), float(d.y), float(d.z)};
+    V3<float> inv{1.0f / df.x, 1.0f / df.y, 1.0f / df.z};
+    const float ex = K * kEps32 * (fabsf(of.x) + fabsf(of.y) + fabsf(of.z) + 1e-3f);
+    const float ed = 2.0f * K * kEps32;
+    float tf;
+    bool amb = false;
+    int h = closest_hit_mixed<SD, CNT>(M, of, df, inv, tf, sRef, sE, ov, tid, w, ex, ed, K, amb);
+    if (CNT) queries++;
+    if (amb) return 2;
+    if (h < 0) return 1;
+    // FP64 recompute of the hit on the chosen triangle (rtKernels.cu:13-41)
+    V3<double> v0, e1, e2;
+    int face;
+    load_tri_verts64(static_cast<const float*>(M.tris), h, v0, e1, e2, face);
+    V3<double> p = cross3(d, e2);
+    double det = dot3(e1, p);
+    double invd = 1.0 / det;
+    V3<double> tv = sub(o, v0);
+    V3<double> q = cross3(tv, e1);
+    double t = dot3(e2, q) * invd;
+    V3<double> n = unit3(cross3(e1, e2));
+    V3<double> hp = axpy(t, d, o);
+    st.P.cosT = dot3(n, V3<double>{-d.x, -d.y, -d.z});
+    st.P.count++;
+    st.P.L += t;
+    if (REC) ids[st.k] = face;
+    double s = 2.0 * dot3(d, n);
+    st.d = unit3(axpy(-s, n, d));
+    st.o = axpy(1e-4, n, hp);
+    st.k++;
+    return st.k >= J.maxB ? 1 : 0;
+}
+
+// Whole mixed-mode path in one call; returns false when a query was ambiguous.
 template <int SD, bool CNT, bool REC>
 __device__ __forceinline__ bool trace_path_mixed(const Job<float>& J, const Job<double>& D, const Mesh<float>& M,
-                                                 int i, int j, int* sRef, float* sE, int* ovRef, float* ovE, int tid,
+                                                 int i, int j, int* sRef, float* sE, Ov<float>::T* ov, int tid,
                                                  int32_t* ids, Work& w, unsigned long long& queries, float K,
                                                  PathOut<double>& out) {
-    double u = (double(i) + 0.5) / double(D.nx);
-    double v = (double(j) + 0.5) / double(D.ny);
-    V3<double> llc{D.llc[0], D.llc[1], D.llc[2]};
-    V3<double> U{D.uv[0], D.uv[1], D.uv[2]}, Vv{D.vv[0], D.vv[1], D.vv[2]};
-    V3<double> o = axpy(v, Vv, axpy(u, U, llc));
-    V3<double> d{D.dir[0], D.dir[1], D.dir[2]};
-    out = PathOut<double>{0, 0.0, 0.0, {0, 0, 0}, {0, 0, 0}};
-    const float* tris = static_cast<const float*>(M.tris);
-    for (int k = 0; k < J.maxB; ++k) {
-        // FP32 copy of the ray; the launch-plane move only shifts the
-        // origin along the ray, which changes no decision
-        const double shift = k == 0 ? J.L0 : 0.0;
-        V3<float> of{float(o.x + shift * d.x), float(o.y + shift * d.y), float(o.z + shift * d.z)};
-        V3<float> df{float(d.x), float(d.y), float(d.z)};
-        V3<float> inv{1.0f / df.x, 1.0f / df.y, 1.0f / df.z};
-        const float ex = K * kEps32 * (fabsf(of.x) + fabsf(of.y) + fabsf(of.z) + 1e-3f);
-        const float ed = 2.0f * K * kEps32;
-        float tf;
-        bool amb = false;
-        int h = closest_hit_mixed<SD, CNT>(M, of, df, inv, tf, sRef, sE, ovRef, ovE, tid, w, ex, ed, K, amb);
-        if (CNT) queries++;
-        if (amb) return false;
-        if (h < 0) break;
-        // FP64 recompute of the hit on the chosen triangle (rtKernels.cu:13-41)
-        V3<double> v0, e1, e2;
-        int face;
-        load_tri_verts64(tris, h, v0, e1, e2, face);
-        V3<double> p = cross3(d, e2);
-        double det = dot3(e1, p);
-        double invd = 1.0 / det;
-        V3<double> tv = sub(o, v0);
-        V3<double> q = cross3(tv, e1);
-        double t = dot3(e2, q) * invd;
-        V3<double> n = unit3(cross3(e1, e2));
-        V3<double> hp = axpy(t, d, o);
-        out.n = n;
-        out.d = d;
-        out.count++;
-        out.L += t;
-        if (REC) ids[k] = face;
-        double s = 2.0 * dot3(d, n);
-        d = unit3(axpy(-s, n, d));
-        o = axpy(1e-4, n, hp);
-    }
-    out.Ld = out.L;
-    return true;
+    RayState<double> st;
+    ray_init_mixed(D, i, j, st);
+    int r = J.maxB > 0 ? 0 : 1;
+    while (r == 0) r = ray_step_mixed<SD, CNT, REC>(J, M, st, sRef, sE, ov, tid, ids, w, queries, K);
+    st.P.Ld = st.P.L;
+    out = st.P;
+    return r != 2;
 }
 
 // Reference FP64 ray with a private stack (mixed-mode re-traces); writes
@@ -612,13 +684,12 @@ template <bool CNT>
 __device__ __forceinline__ void fp64_retrace(const Job<double>& J, const Mesh<double>& M, int i, int j, int32_t* ids,
                                              Work& w, unsigned long long& queries, PathOut<double>& P, double& re,
                                              double& im) {
-    int ovRef[kMaxStack];
-    double ovE[kMaxStack];
+    Ov<double>::T ov[kMaxStack];
     if (ids) {
-        P = trace_path<double, 0, CNT, true>(J, M, i, j, nullptr, nullptr, ovRef, ovE, 0, ids, w, queries);
+        P = trace_path<double, 0, CNT, true>(J, M, i, j, nullptr, nullptr, ov, 0, ids, w, queries);
         for (int k = P.count; k < J.maxB; ++k) ids[k] = -1;
     } else {
-        P = trace_path<double, 0, CNT, false>(J, M, i, j, nullptr, nullptr, ovRef, ovE, 0, nullptr, w, queries);
+        P = trace_path<double, 0, CNT, false>(J, M, i, j, nullptr, nullptr, ov, 0, nullptr, w, queries);
     }
     po_term<double>(J, P, false, re, im);
 }
@@ -628,13 +699,31 @@ __device__ __forceinline__ void fix_add(unsigned long long* p, double v, double 
     atomicAdd(p, static_cast<unsigned long long>(q));
 }
 
-template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST>
+// Per-ray result slot of the ray-pool variant: the ray's (re, im) term in
+// the precision the accumulator adds it in.
+template <typename R, int MODE> struct SlotOf {
+    using T = double;
+};
+template <> struct SlotOf<float, 0> {
+    using T = float;
+};
+constexpr int kPoolRays = 1024;  // rays per pool batch (4 passes of 16x16)
+
+// POOL: lanes are refilled from the tile's ray pool as soon as their ray
+// ends (one closest-hit query per loop iteration, so lanes of a wave never
+// wait for a longer path); each ray's term goes to its own LDS slot and the
+// slots are reduced per 16x16 pass exactly as without the pool.
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL>
 __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
+    using ST = typename SlotOf<R, MODE>::T;
+    using RS = typename std::conditional<MODE == 1, double, R>::type;  // ray state precision
     __shared__ int sRef[SD * kWgSize];
     __shared__ R sE[SD * kWgSize];
     __shared__ Acc<R, MODE> red[kWgSize / 64];
+    __shared__ ST sRes[POOL ? 2 * kPoolRays : 1];
     __shared__ int64_t sNext;
     __shared__ unsigned sPrev;
+    __shared__ unsigned sPool;
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -642,8 +731,7 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
     const int qx = (wave & 1) * 8 + (lane & 7);
     const int qy = (wave >> 1) * 8 + (lane >> 3);
     unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0, cRetrace = 0;
-    int ovRef[kMaxStack - SD];
-    R ovE[kMaxStack - SD];
+    typename Ov<R>::T ov[kMaxStack - SD];
 
     // Persistent mode pops tiles from a global counter. The pop for the next
     // tile is made by thread 0 in the same block that bumps the job's
@@ -653,11 +741,16 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
     // the pop for lanes 1..63 of wave 0 (seen in the generated code as a
     // loop that never ends).
     int64_t tile = a.tileOffset + blockIdx.x;
+    // The pool counter is never reset: every lane ends each batch with
+    // exactly one failed fetch, so a batch of n rays advances it by
+    // n + kWgSize and every thread can track the base itself.
+    unsigned poolBase = 0;
+    if (POOL && tid == 0) sPool = 0;
     if (PERSIST) {
         if (tid == 0) sNext = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
-        __syncthreads();
-        tile = uniform64(sNext);
     }
+    if (PERSIST || POOL) __syncthreads();
+    if (PERSIST) tile = uniform64(sNext);
     while (tile < a.totalTiles) {
 
         const int jid = find_job(&a.jobs[0].tileBase, sizeof(Job<R>), a.nJobs, tile);
@@ -667,83 +760,189 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
         const int ts = a.tileSide;
         const int tx = tl % J.tilesX, ty = tl / J.tilesX;
 
-        Acc<R, MODE> acc = Acc<R, MODE>::zero();
-        // a tile of side ts is covered by (ts/16)^2 passes of 16x16 rays
-        for (int py = 0; py < ts; py += 16) {
-            for (int px = 0; px < ts; px += 16) {
-                const int i = J.i0 + tx * ts + px + qx;
-                const int j = J.j0 + ty * ts + py + qy;
-                if (i >= J.nx || j >= J.ny) continue;
-                if (CNT) cRays++;
-                const int ray = j * J.nx + i;
-                int32_t* ids = REC ? a.recIds + size_t(ray) * J.maxB : nullptr;
+        using AccT = Acc<R, MODE>;
+        AccT* parts = static_cast<AccT*>(a.partials);
+        if constexpr (POOL) {
+            const int perSide = ts >> 4;
+            const int nPass = perSide * perSide;
+            for (int b0 = 0; b0 < nPass; b0 += kPoolRays / 256) {
+                const int nb = min(kPoolRays / 256, nPass - b0);
+                const unsigned nRays = static_cast<unsigned>(nb) << 8;
+                // slot r -> pass b0 + r/256, and within the pass the same
+                // 8x8-per-wave layout as the one-ray-per-lane path
+                unsigned r = 0;
+                int ri = 0, rj = 0;
+                auto fetch = [&]() -> bool {
+                    for (;;) {
+                        r = atomicAdd(&sPool, 1u) - poolBase;
+                        if (r >= nRays) return false;
+                        const int pIdx = b0 + static_cast<int>(r >> 8);
+                        const int q = static_cast<int>(r & 255u);
+                        const int w2 = q >> 6, l2 = q & 63;
+                        ri = J.i0 + tx * ts + (pIdx % perSide) * 16 + (w2 & 1) * 8 + (l2 & 7);
+                        rj = J.j0 + ty * ts + (pIdx / perSide) * 16 + (w2 >> 1) * 8 + (l2 >> 3);
+                        if (ri < J.nx && rj < J.ny) return true;
+                        sRes[2 * r] = ST(0);
+                        sRes[2 * r + 1] = ST(0);
+                    }
+                };
+                RayState<RS> st;
                 Work w{0, 0};
-                R re = 0, im = 0;
-                double dre = 0.0, dimg = 0.0;
-                PathOut<R> P;
-                PathOut<double> PD;
-                bool ok = true;
-                if constexpr (MODE == 1) {
-                    const Job<double>& D = a.jobs64[jid];
-                    ok = trace_path_mixed<SD, CNT, REC>(J, D, M, i, j, sRef, sE, ovRef, ovE, tid, ids, w, cQueries,
-                                                       a.errK, PD);
-                    if (ok) po_term<double>(D, PD, false, dre, dimg);
-                    P.count = PD.count;
-                    P.Ld = PD.Ld;
-                } else {
-                    P = trace_path<R, SD, CNT, REC>(J, M, i, j, sRef, sE, ovRef, ovE, tid, ids, w, cQueries);
-                    po_term<R>(J, P, false, re, im);
+                bool alive = fetch();
+                auto start = [&]() {
+                    if (CNT) cRays++;
+                    if constexpr (MODE == 1) ray_init_mixed(a.jobs64[jid], ri, rj, st);
+                    else ray_init(J, ri, rj, st);
+                };
+                if (alive) start();
+                while (__ballot(alive) != 0ull) {
+                    if (alive) {
+                        bool done, ok = true;
+                        if constexpr (MODE == 1) {
+                            const int rc = ray_step_mixed<SD, CNT, false>(J, M, st, sRef, sE, ov, tid, nullptr,
+                                                                          w, cQueries, a.errK);
+                            done = rc != 0;
+                            ok = rc != 2;
+                        } else {
+                            done = ray_step<R, SD, CNT, false>(J, M, st, sRef, sE, ov, tid, nullptr, w,
+                                                               cQueries);
+                        }
+                        if (done) {
+                            ST re = ST(0), im = ST(0);
+                            if constexpr (MODE == 1) {
+                                if (ok) {
+                                    st.P.Ld = st.P.L;
+                                    po_term<double>(a.jobs64[jid], st.P, false, re, im);
+                                    if (CNT) cBounces += st.P.count;
+                                } else {
+                                    if (CNT) cRetrace++;
+                                    const int ray = rj * J.nx + ri;
+                                    unsigned qi = atomicAdd(a.queueCount, 1u);
+                                    if (qi < a.queueCap) {
+                                        a.queue[qi] = QEntry{jid, ray};
+                                    } else {
+                                        // queue full: re-trace inline (same FP64 path, same fixed-point sum)
+                                        const Job<double>& J64 = a.jobs64[jid];
+                                        PathOut<double> P64;
+                                        double r64, i64;
+                                        Work w2{0, 0};
+                                        unsigned long long q2 = 0;
+                                        fp64_retrace<false>(J64, a.meshes64[J64.mesh], ri, rj, nullptr, w2, q2, P64,
+                                                            r64, i64);
+                                        fix_add(&a.jobFix[2 * jid], r64, J64.fixScale);
+                                        fix_add(&a.jobFix[2 * jid + 1], i64, J64.fixScale);
+                                    }
+                                }
+                            } else {
+                                po_term<R>(J, st.P, false, re, im);
+                                if (CNT) cBounces += st.P.count;
+                            }
+                            sRes[2 * r] = re;
+                            sRes[2 * r + 1] = im;
+                            alive = fetch();
+                            if (alive) start();
+                        }
+                    }
                 }
                 if (CNT) {
                     cVisits += w.visits;
                     cTests += w.tests;
                 }
-                if constexpr (MODE == 1) {
-                    if (!ok) {
-                        if (CNT) cRetrace++;
-                        if (REC) a.recFlag[ray] = 1;
-                        unsigned q = atomicAdd(a.queueCount, 1u);
-                        if (q < a.queueCap) {
-                            a.queue[q] = QEntry{jid, ray};
-                        } else {
-                            // queue full: re-trace inline (same FP64 path, same fixed-point sum)
-                            const Job<double>& J64 = a.jobs64[jid];
-                            PathOut<double> P64;
-                            double r64, i64;
-                            Work w2{0, 0};
-                            unsigned long long q2 = 0;
-                            fp64_retrace<false>(J64, a.meshes64[J64.mesh], i, j, REC ? ids : nullptr, w2, q2, P64,
-                                                r64, i64);
-                            fix_add(&a.jobFix[2 * jid], r64, J64.fixScale);
-                            fix_add(&a.jobFix[2 * jid + 1], i64, J64.fixScale);
-                            if (REC) {
-                                a.recCount[ray] = P64.count;
-                                a.recL[ray] = P64.Ld;
-                                a.recReIm[2 * size_t(ray)] = r64;
-                                a.recReIm[2 * size_t(ray) + 1] = i64;
-                            }
-                        }
-                        continue;
+                poolBase += nRays + kWgSize;
+                __syncthreads();
+                for (int pp = 0; pp < nb; ++pp) {
+                    AccT acc = AccT::zero();
+                    const ST sr = sRes[2 * ((pp << 8) + tid)], si = sRes[2 * ((pp << 8) + tid) + 1];
+                    if constexpr (MODE == 1) acc.add_ray64(sr, si);
+                    else acc.add_ray(sr, si);
+                    const AccT psum = wg_sum(acc, red, tid);
+                    const int pIdx = b0 + pp;
+                    const int sx = (tx * ts + (pIdx % perSide) * 16) >> 4, sy = (ty * ts + (pIdx / perSide) * 16) >> 4;
+                    if (tid == 0 && sx < J.passX && sy < J.passY)
+                        parts[J.passBase + int64_t(sy) * J.passX + sx] = psum;
+                }
+            }
+        } else
+        // a tile of side ts is covered by (ts/16)^2 passes of 16x16 rays; each
+        // pass is reduced on its own and stored at its place in the job's
+        // pass grid
+        for (int py = 0; py < ts; py += 16) {
+            for (int px = 0; px < ts; px += 16) {
+                AccT acc = AccT::zero();
+                const int i = J.i0 + tx * ts + px + qx;
+                const int j = J.j0 + ty * ts + py + qy;
+                do {
+                    if (i >= J.nx || j >= J.ny) break;
+                    if (CNT) cRays++;
+                    const int ray = j * J.nx + i;
+                    int32_t* ids = REC ? a.recIds + size_t(ray) * J.maxB : nullptr;
+                    Work w{0, 0};
+                    R re = 0, im = 0;
+                    double dre = 0.0, dimg = 0.0;
+                    PathOut<R> P;
+                    PathOut<double> PD;
+                    bool ok = true;
+                    if constexpr (MODE == 1) {
+                        const Job<double>& D = a.jobs64[jid];
+                        ok = trace_path_mixed<SD, CNT, REC>(J, D, M, i, j, sRef, sE, ov, tid, ids, w, cQueries,
+                                                           a.errK, PD);
+                        if (ok) po_term<double>(D, PD, false, dre, dimg);
+                        P.count = PD.count;
+                        P.Ld = PD.Ld;
+                    } else {
+                        P = trace_path<R, SD, CNT, REC>(J, M, i, j, sRef, sE, ov, tid, ids, w, cQueries);
+                        po_term<R>(J, P, false, re, im);
                     }
-                }
-                if (CNT) cBounces += P.count;
-                if (REC) {
-                    for (int k = P.count; k < J.maxB; ++k) ids[k] = -1;
-                    a.recCount[ray] = P.count;
-                    a.recL[ray] = MODE == 1 ? P.Ld : double(P.L);
-                    a.recReIm[2 * size_t(ray)] = MODE == 1 ? dre : double(re);
-                    a.recReIm[2 * size_t(ray) + 1] = MODE == 1 ? dimg : double(im);
-                }
-                if constexpr (MODE == 1) acc.add_ray64(dre, dimg);
-                else acc.add_ray(re, im);
+                    if (CNT) {
+                        cVisits += w.visits;
+                        cTests += w.tests;
+                    }
+                    if constexpr (MODE == 1) {
+                        if (!ok) {
+                            if (CNT) cRetrace++;
+                            if (REC) a.recFlag[ray] = 1;
+                            unsigned q = atomicAdd(a.queueCount, 1u);
+                            if (q < a.queueCap) {
+                                a.queue[q] = QEntry{jid, ray};
+                            } else {
+                                // queue full: re-trace inline (same FP64 path, same fixed-point sum)
+                                const Job<double>& J64 = a.jobs64[jid];
+                                PathOut<double> P64;
+                                double r64, i64;
+                                Work w2{0, 0};
+                                unsigned long long q2 = 0;
+                                fp64_retrace<false>(J64, a.meshes64[J64.mesh], i, j, REC ? ids : nullptr, w2, q2, P64,
+                                                    r64, i64);
+                                fix_add(&a.jobFix[2 * jid], r64, J64.fixScale);
+                                fix_add(&a.jobFix[2 * jid + 1], i64, J64.fixScale);
+                                if (REC) {
+                                    a.recCount[ray] = P64.count;
+                                    a.recL[ray] = P64.Ld;
+                                    a.recReIm[2 * size_t(ray)] = r64;
+                                    a.recReIm[2 * size_t(ray) + 1] = i64;
+                                }
+                            }
+                            break;
+                        }
+                    }
+                    if (CNT) cBounces += P.count;
+                    if (REC) {
+                        for (int k = P.count; k < J.maxB; ++k) ids[k] = -1;
+                        a.recCount[ray] = P.count;
+                        a.recL[ray] = MODE == 1 ? P.Ld : double(P.L);
+                        a.recReIm[2 * size_t(ray)] = MODE == 1 ? dre : double(re);
+                        a.recReIm[2 * size_t(ray) + 1] = MODE == 1 ? dimg : double(im);
+                    }
+                    if constexpr (MODE == 1) acc.add_ray64(dre, dimg);
+                    else acc.add_ray(re, im);
+                } while (false);
+                const AccT psum = wg_sum(acc, red, tid);
+                const int sx = (tx * ts + px) >> 4, sy = (ty * ts + py) >> 4;
+                if (tid == 0 && sx < J.passX && sy < J.passY) parts[J.passBase + int64_t(sy) * J.passX + sx] = psum;
             }
         }
 
-        Acc<R, MODE> tsum = wg_sum(acc, red, tid);
-        using AccT = Acc<R, MODE>;
-        AccT* parts = static_cast<AccT*>(a.partials);
         if (tid == 0) {
-            parts[tile] = tsum;
             __threadfence();
             sPrev = atomicAdd(&a.jobDone[jid], 1u);
             if (PERSIST) sNext = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
@@ -755,7 +954,8 @@ __global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
             // last tile of this job: fixed-order reduction of its partials
             __threadfence();
             AccT s = AccT::zero();
-            for (int q = tid; q < J.nTiles; q += kWgSize) s.add(parts[J.tileBase + q]);
+            const int nPass = J.passX * J.passY;
+            for (int q = tid; q < nPass; q += kWgSize) s.add(parts[J.passBase + q]);
             AccT tot = wg_sum(s, red, tid);
             if (tid == 0) {
                 a.jobOut[2 * jid] = tot.dre();
@@ -828,19 +1028,25 @@ __global__ void zero_words(unsigned* p, size_t n) {
     if (i < n) p[i] = 0u;
 }
 
-template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST>
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL>
 hipError_t launch_one(const TraceArgs<R>& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST>), dim3(grid), dim3(kWgSize), 0, s, a);
+    hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST, POOL>), dim3(grid), dim3(kWgSize), 0, s, a);
     return hipGetLastError();
 }
 
+template <typename R, int MODE, int SD, bool POOL>
+hipError_t launch_pool(const TraceArgs<R>& a, bool cnt, bool persist, int grid, hipStream_t s) {
+    if (cnt) return persist ? launch_one<R, MODE, SD, false, true, true, POOL>(a, grid, s)
+                            : launch_one<R, MODE, SD, false, true, false, POOL>(a, grid, s);
+    return persist ? launch_one<R, MODE, SD, false, false, true, POOL>(a, grid, s)
+                   : launch_one<R, MODE, SD, false, false, false, POOL>(a, grid, s);
+}
+
 template <typename R, int MODE, int SD>
-hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, int grid, hipStream_t s) {
-    if (rec) return launch_one<R, MODE, SD, true, false, false>(a, grid, s);
-    if (cnt) return persist ? launch_one<R, MODE, SD, false, true, true>(a, grid, s)
-                            : launch_one<R, MODE, SD, false, true, false>(a, grid, s);
-    return persist ? launch_one<R, MODE, SD, false, false, true>(a, grid, s)
-                   : launch_one<R, MODE, SD, false, false, false>(a, grid, s);
+hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, bool pool, int grid, hipStream_t s) {
+    if (rec) return launch_one<R, MODE, SD, true, false, false, false>(a, grid, s);
+    return pool ? launch_pool<R, MODE, SD, true>(a, cnt, persist, grid, s)
+                : launch_pool<R, MODE, SD, false>(a, cnt, persist, grid, s);
 }
 
 }  // namespace
@@ -848,22 +1054,23 @@ hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, in
 // Host entry used by the driver. sd: traversal stack entries kept in LDS
 // per ray (8, 16 or 32); deeper entries spill to scratch.
 template <typename R>
-hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, int grid,
-                        hipStream_t s) {
+hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool,
+                        int grid, hipStream_t s) {
     if constexpr (std::is_same_v<R, float>) {
         if (mode == 1) {
-            if (sd <= 8) return launch_sd<float, 1, 8>(a, rec, cnt, persist, grid, s);
-            if (sd <= 16) return launch_sd<float, 1, 16>(a, rec, cnt, persist, grid, s);
-            return launch_sd<float, 1, 32>(a, rec, cnt, persist, grid, s);
+            if (sd <= 8) return launch_sd<float, 1, 8>(a, rec, cnt, persist, pool, grid, s);
+            if (sd <= 16) return launch_sd<float, 1, 16>(a, rec, cnt, persist, pool, grid, s);
+            return launch_sd<float, 1, 32>(a, rec, cnt, persist, pool, grid, s);
         }
     }
-    if (sd <= 8) return launch_sd<R, 0, 8>(a, rec, cnt, persist, grid, s);
-    if (sd <= 16) return launch_sd<R, 0, 16>(a, rec, cnt, persist, grid, s);
-    return launch_sd<R, 0, 32>(a, rec, cnt, persist, grid, s);
+    if (sd <= 8) return launch_sd<R, 0, 8>(a, rec, cnt, persist, pool, grid, s);
+    if (sd <= 16) return launch_sd<R, 0, 16>(a, rec, cnt, persist, pool, grid, s);
+    return launch_sd<R, 0, 32>(a, rec, cnt, persist, pool, grid, s);
 }
 
-template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool, bool, bool, int, hipStream_t);
-template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, int, hipStream_t);
+template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool, bool, bool, bool, int, hipStream_t);
+template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, bool, int,
+                                         hipStream_t);
 
 hipError_t launch_retrace(const TraceArgs<float>& a, bool rec, bool cnt, int grid, hipStream_t s) {
     if (rec) hipLaunchKernelGGL((retrace_kernel<true, false>), dim3(grid), dim3(kWgSize), 0, s, a);
@@ -886,22 +1093,25 @@ hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <typename R, int MODE, int SD> int occ_one() {
+template <typename R, int MODE, int SD> int occ_one(bool pool) {
     int n = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &n, reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true>), kWgSize, 0);
+    const void* f = pool ? reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, true>)
+                         : reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, false>);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, kWgSize, 0);
     return e == hipSuccess ? n : 1;
 }
 
 // Resident workgroups per CU of the persistent kernel variant.
-template <typename R> int trace_occupancy(int mode, int sd) {
+template <typename R> int trace_occupancy(int mode, int sd, bool pool) {
     if constexpr (std::is_same_v<R, float>) {
-        if (mode == 1) return sd <= 8 ? occ_one<float, 1, 8>() : (sd <= 16 ? occ_one<float, 1, 16>() : occ_one<float, 1, 32>());
+        if (mode == 1)
+            return sd <= 8 ? occ_one<float, 1, 8>(pool)
+                           : (sd <= 16 ? occ_one<float, 1, 16>(pool) : occ_one<float, 1, 32>(pool));
     }
-    return sd <= 8 ? occ_one<R, 0, 8>() : (sd <= 16 ? occ_one<R, 0, 16>() : occ_one<R, 0, 32>());
+    return sd <= 8 ? occ_one<R, 0, 8>(pool) : (sd <= 16 ? occ_one<R, 0, 16>(pool) : occ_one<R, 0, 32>(pool));
 }
-template int trace_occupancy<float>(int, int);
-template int trace_occupancy<double>(int, int);
+template int trace_occupancy<float>(int, int, bool);
+template int trace_occupancy<double>(int, int, bool);
 
 // Size in bytes of one per-tile partial for (precision mode).
 size_t partial_bytes(bool fp64, int mode) {

@@ -219,26 +219,57 @@ def test_graph_replay_matches_eager(ctx, sphere, tri_duct):
         pl.close()
 
 
-def test_tuning_variants(ctx, tri_duct):
+@pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_MIXED, S.SBR_FP64])
+def test_tuning_variants(ctx, tri_duct, prec):
+    """Partial sums are kept per 16x16 ray pass and reduced in a fixed order,
+    so every launch shape and both methods give the same sigma to the bit."""
     v, f, mesh, s32, s64 = tri_duct
     p = S.SweepParams.make(**dict(C2S, theta_samples=5, phi_samples=5))
-    jobs = S.sweep_jobs(p, mesh, S.SBR_FP32)
-    pl = ctx.plan(jobs, S.SBR_FP32, S.SBR_BATCHED)
+    jobs = S.sweep_jobs(p, mesh, prec)
+    pl = ctx.plan(jobs, prec, S.SBR_BATCHED)
     pl.run()
     pl.wait()
     z0, _, _ = pl.results()
-    for sd in (8, 16, 32):
-        for wg in (1, 2, 0):
-            _step(f"fp32 batched sd={sd} wg={wg}")
-            pl.tune(wg_per_cu=wg, lds_stack=sd)
-            pl.run()
-            pl.wait()
-            z, _, _ = pl.results()
-            assert np.array_equal(z, z0), (sd, wg)
-    for ts in (32, 64):
-        pl.tune(tile_rays=ts)
+    for pool in (1, 2):
+        for ts in (16, 32, 64):
+            for sd in (8, 16, 32):
+                for wg in ((1, 0) if sd != 16 else (2,)):
+                    _step(f"precision {prec} batched pool={pool} tile={ts} sd={sd} wg={wg}")
+                    pl.tune(wg_per_cu=wg, lds_stack=sd, tile_rays=ts, ray_pool=pool)
+                    pl.run()
+                    pl.wait()
+                    z, _, _ = pl.results()
+                    assert np.array_equal(z, z0), (pool, ts, sd, wg)
+    pl.close()
+    pa = ctx.plan(jobs, prec, S.SBR_PER_ANGLE)
+    for pool in (1, 2):
+        for ts in (16, 64):
+            _step(f"precision {prec} per-angle pool={pool} tile={ts}")
+            pa.tune(tile_rays=ts, ray_pool=pool)
+            pa.run()
+            pa.wait()
+            z, _, _ = pa.results()
+            assert np.array_equal(z, z0), (pool, ts)
+    pa.close()
+
+
+def test_autotune(ctx, tri_duct):
+    v, f, mesh, s32, s64 = tri_duct
+    p = S.SweepParams.make(**dict(C2S, theta_samples=4, phi_samples=6))
+    jobs = S.sweep_jobs(p, mesh, S.SBR_MIXED)
+    for method in (S.SBR_BATCHED, S.SBR_PER_ANGLE):
+        pl = ctx.plan(jobs, S.SBR_MIXED, method)
+        pl.run()
+        pl.wait()
+        z0, _, _ = pl.results()
+        _step(f"autotune method {method}")
+        best = pl.autotune(sample_jobs=8 if method == S.SBR_BATCHED else 0, repeats=1)
+        print("   ", best, flush=True)
+        assert best["candidates"] == (36 if method == S.SBR_BATCHED else 24)
+        assert best["tile_rays"] in (16, 32, 64) and best["lds_stack"] in (8, 16, 32) and best["best_ms"] > 0
+        assert best["ray_pool"] in (1, 2)
         pl.run()
         pl.wait()
         z, _, _ = pl.results()
-        np.testing.assert_allclose(z, z0, rtol=1e-4, atol=1e-5 * np.abs(z0).max())
-    pl.close()
+        assert np.array_equal(z, z0)
+        pl.close()
