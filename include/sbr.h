@@ -127,6 +127,10 @@ int sbr_ctx_cu_count(const sbr_ctx* ctx, int* out);
 int sbr_mesh_create(sbr_ctx* ctx, const float* xyz, int n_verts, const uint32_t* idx, int n_tris, double scale,
                     const sbr_bvh_options* opts, int* mesh_id);
 int sbr_mesh_info_get(sbr_ctx* ctx, int mesh_id, int precision, sbr_mesh_info* out);
+/* Free a mesh's device data (the reference frees its BVH arrays at the end
+ * of runRcsApp, src/app/rtRcsEntry.cu:322-323). Its id is not reused; plans
+ * and adjoints on it fail with an error afterwards. */
+int sbr_mesh_destroy(sbr_ctx* ctx, int mesh_id);
 /* BVH-order -> original face index permutation (n_triangles ints). */
 int sbr_mesh_permutation(sbr_ctx* ctx, int mesh_id, int precision, int32_t* perm);
 /* Reference-layout BVH export for parity checks: bounds 6*n_nodes doubles,
@@ -208,6 +212,21 @@ int sbr_sweep(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precisio
  * (SBR_MIXED) 1 where the ray was re-traced in FP64. */
 int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int angle, int precision, int32_t* hit_ids,
                    int32_t* bounces, double* path_len, double* reim_ray, int8_t* retraced);
+
+/* Frozen-path adjoint (no counterpart in the reference; it differentiates
+ * the per-angle sigma of runSweep, src/app/sweep.cu:228-231, built from
+ * poKernels.cu:75-110 over rtKernels.cu:105-158 paths). For each job: a
+ * mixed-precision forward pass (FP64 outcome per ray) records every ray's
+ * hit faces; an FP64 kernel then recomputes the bounce geometry from those
+ * faces and accumulates, in reverse, d sigma / d (vertex coordinates) and
+ * d sigma / d f. All jobs must be on mesh_id. Outputs (any may be NULL):
+ * sigma[n_jobs]; dloss_dxyz[n_verts*3] = sum_j dloss_dsigma[j] *
+ * d sigma_j / d xyz, with xyz the coordinates passed to sbr_mesh_create
+ * (before the mesh scale); dsigma_dfreq[n_jobs] = d sigma_j / d f_j [1/Hz].
+ * Gradients are accumulated with FP64 atomics: reproducible to rounding,
+ * not bit for bit. */
+int sbr_adjoint(sbr_ctx* ctx, int mesh_id, const sbr_job* jobs, int n_jobs, const double* dloss_dsigma,
+                double* sigma, double* dloss_dxyz, double* dsigma_dfreq);
 
 #ifdef __cplusplus
 }

@@ -413,14 +413,9 @@ template <typename R> struct Frame {
     V3<R> llc, uvec, vvec, dir;  // dir = ray direction (= -radar direction)
 };
 
-// sweep.cu:336-364
-template <typename R>
-Frame<R> frame(double thS, double thE, int nTh, double phS, double phE, int nPh, double grid, int idx) {
+// sweep.cu:349-364 for given angles in degrees
+template <typename R> Frame<R> frame_deg(R thDeg, R phDeg, R g) {
     const R pi = R(M_PI);
-    int t = idx / nPh, p = idx % nPh;
-    R ts = R(thS), te = R(thE), ps = R(phS), pe = R(phE), g = R(grid);
-    R thDeg = ts + (nTh > 1 ? t * (te - ts) / (nTh - 1) : R(0));
-    R phDeg = ps + (nPh > 1 ? p * (pe - ps) / (nPh - 1) : R(0));
     R th = thDeg * pi / R(180.0);
     R ph = phDeg * pi / R(180.0);
     V3<R> dir = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
@@ -439,6 +434,16 @@ Frame<R> frame(double thS, double thE, int nTh, double phS, double phE, int nPh,
     f.vvec = v;
     f.dir = neg(dir);
     return f;
+}
+
+// sweep.cu:336-364
+template <typename R>
+Frame<R> frame(double thS, double thE, int nTh, double phS, double phE, int nPh, double grid, int idx) {
+    int t = idx / nPh, p = idx % nPh;
+    R ts = R(thS), te = R(thE), ps = R(phS), pe = R(phE), g = R(grid);
+    R thDeg = ts + (nTh > 1 ? t * (te - ts) / (nTh - 1) : R(0));
+    R phDeg = ps + (nPh > 1 ? p * (pe - ps) / (nPh - 1) : R(0));
+    return frame_deg<R>(thDeg, phDeg, g);
 }
 
 struct Params {
@@ -511,6 +516,89 @@ RayResult<R> trace_ray(const Scene<R>& sc, const Frame<R>& fr, const Params& p, 
         }
     }
     return rr;
+}
+
+// ------------------------------------------------------- frozen paths (FP64)
+// The ray of rtKernels.cu:113-158 in FP64 with its hit faces given (no
+// traversal): each bounce's distance is the Moller-Trumbore distance on the
+// recorded face, computed with hit_tri's operations; the term is that of
+// poKernels.cu:75-110. Used as the finite-difference reference of the
+// adjoint (derivatives of sigma along frozen paths).
+struct FrozenMesh {
+    const double* verts;  // n_verts*3, scaled
+    const uint32_t* idx;  // n_tris*3
+};
+
+inline V3<double> fvtx(const FrozenMesh& m, int f, int k) {
+    const uint32_t vi = m.idx[3 * f + k];
+    return mk(m.verts[3 * size_t(vi)], m.verts[3 * size_t(vi) + 1], m.verts[3 * size_t(vi) + 2]);
+}
+
+struct FrozenJob {
+    Frame<double> fr;
+    Phys<double> ph;
+    int nx, ny, maxB;
+    double refl;
+};
+
+inline FrozenJob frozen_job(const double* pj, const int* ij) {
+    // pj: theta_deg, phi_deg, grid, freq, refl; ij: nx, ny, maxB
+    FrozenJob J;
+    J.fr = frame_deg<double>(pj[0], pj[1], pj[2]);
+    Params p{};
+    p.grid = pj[2];
+    p.freq = pj[3];
+    p.refl = pj[4];
+    p.nx = ij[0];
+    p.ny = ij[1];
+    p.maxB = ij[2];
+    J.ph = phys<double>(p);
+    J.nx = ij[0];
+    J.ny = ij[1];
+    J.maxB = ij[2];
+    J.refl = pj[4];
+    return J;
+}
+
+inline void frozen_ray(const FrozenMesh& m, const FrozenJob& J, int ray, const int32_t* fids, int cnt, double& re,
+                       double& im) {
+    using R = double;
+    const int i = ray % J.nx, j = ray / J.nx;
+    R u = (R(i) + R(0.5)) / R(J.nx);
+    R v = (R(j) + R(0.5)) / R(J.ny);
+    V3<R> o = axpy<R, false>(v, J.fr.vvec, axpy<R, false>(u, J.fr.uvec, J.fr.llc));
+    V3<R> d = J.fr.dir;
+    V3<R> nl{0, 0, 0}, dl{0, 0, 0};
+    R L = 0;
+    for (int b = 0; b < cnt; ++b) {
+        V3<R> v0 = fvtx(m, fids[b], 0);
+        V3<R> e1 = fvtx(m, fids[b], 1) - v0, e2 = fvtx(m, fids[b], 2) - v0;
+        V3<R> pv = cross<R, false>(d, e2);
+        R det = dot<R, false>(e1, pv);
+        R inv = R(1.0) / det;
+        V3<R> tv = o - v0;
+        V3<R> q = cross<R, false>(tv, e1);
+        R t = dot<R, false>(e2, q) * inv;
+        V3<R> n = unit<R, false>(cross<R, false>(e1, e2));
+        V3<R> hp = axpy<R, false>(t, d, o);
+        nl = n;
+        dl = d;
+        L += t;
+        R s = R(2.0) * dot<R, false>(d, n);
+        d = unit<R, false>(axpy<R, false>(-s, n, d));
+        o = axpy<R, false>(R(1e-4), n, hp);
+    }
+    re = 0;
+    im = 0;
+    if (cnt <= 0) return;
+    R refl = R(1.0);
+    for (int b = 0; b < cnt; ++b) refl *= R(J.refl);
+    R cosT = dot<R, false>(nl, neg(dl));
+    if (!(cosT > R(0))) return;
+    R phase = std::fmod(R(2.0) * J.ph.k * L, J.ph.twoPi);
+    R mag = J.ph.pre * cosT * refl;
+    re = -mag * std::sin(phase);
+    im = mag * std::cos(phase);
 }
 
 }  // namespace orc
@@ -767,6 +855,126 @@ int orc_sweep(const orc_scene* o, const double* pd, const int* pi, int fused, co
         tot[0] += c[0]; tot[1] += c[1]; tot[2] += c[2];
     }
     if (ctr3) { ctr3[0] = tot[0]; ctr3[1] = tot[1]; ctr3[2] = tot[2]; }
+    return 0;
+}
+
+// Complex sum of one job along frozen paths (ids: rays*maxB original face
+// ids, cnt: rays). pj: theta_deg, phi_deg, grid, freq, refl; ij: nx, ny, maxB.
+int orc_frozen_sum(const double* verts, const uint32_t* idx, const double* pj, const int* ij, const int32_t* ids,
+                   const int32_t* cnt, double* reim) {
+    FrozenMesh m{verts, idx};
+    FrozenJob J = frozen_job(pj, ij);
+    const int nr = J.nx * J.ny;
+    double sr = 0, si = 0;
+    for (int r = 0; r < nr; ++r) {
+        double a, b;
+        frozen_ray(m, J, r, ids + size_t(r) * J.maxB, cnt[r], a, b);
+        sr += a;
+        si += b;
+    }
+    reim[0] = sr;
+    reim[1] = si;
+    return 0;
+}
+
+// Central differences of sum_j w_j sigma_j along frozen paths with respect
+// to vertex coordinates coords[c] = 3*vertex + axis (step h in the scaled
+// coordinates; with richardson != 0 the O(h^4) combination
+// (4 D(h/2) - D(h)) / 3). Only rays whose recorded path touches the vertex
+// are re-evaluated; the others cancel exactly. pj: 5 doubles per job, ij: 3 ints
+// per job; ids/cnt: per job, rays*maxB / rays. Outputs grad[nCoords] and
+// sigma[nJobs] (may be NULL).
+int orc_frozen_fd(const double* verts, int nVerts, const uint32_t* idx, int nTris, int nJobs, const double* pj,
+                  const int* ij, const int32_t* ids, const int32_t* cnt, const double* w, int nCoords,
+                  const int* coords, double h, int richardson, double* grad, double* sigma) {
+    FrozenMesh m{verts, idx};
+    std::vector<FrozenJob> J(nJobs);
+    std::vector<size_t> rayBase(nJobs + 1, 0), idBase(nJobs + 1, 0);
+    for (int k = 0; k < nJobs; ++k) {
+        J[k] = frozen_job(pj + 5 * k, ij + 3 * k);
+        rayBase[k + 1] = rayBase[k] + size_t(J[k].nx) * J[k].ny;
+        idBase[k + 1] = idBase[k] + size_t(J[k].nx) * J[k].ny * J[k].maxB;
+    }
+    // base sums
+    std::vector<double> S(2 * nJobs, 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < nJobs; ++k) {
+        int ijk[3] = {J[k].nx, J[k].ny, J[k].maxB};
+        orc_frozen_sum(verts, idx, pj + 5 * k, ijk, ids + idBase[k], cnt + rayBase[k], &S[2 * k]);
+    }
+    if (sigma)
+        for (int k = 0; k < nJobs; ++k) sigma[k] = 4.0 * M_PI * (S[2 * k] * S[2 * k] + S[2 * k + 1] * S[2 * k + 1]);
+    // rays touching each target vertex
+    std::vector<int> target(nVerts, -1);
+    std::vector<int> tv;
+    for (int c = 0; c < nCoords; ++c) {
+        int v = coords[c] / 3;
+        if (v < 0 || v >= nVerts) return 1;
+        if (target[v] < 0) {
+            target[v] = int(tv.size());
+            tv.push_back(v);
+        }
+    }
+    std::vector<std::vector<std::pair<int, int>>> touch(tv.size());
+    for (int k = 0; k < nJobs; ++k) {
+        const size_t nr = rayBase[k + 1] - rayBase[k];
+        for (size_t r = 0; r < nr; ++r) {
+            const int c = cnt[rayBase[k] + r];
+            const int32_t* f = ids + idBase[k] + r * J[k].maxB;
+            int last = -1;
+            for (int b = 0; b < c; ++b) {
+                if (f[b] < 0 || f[b] >= nTris) return 2;
+                for (int q = 0; q < 3; ++q) {
+                    int t = target[idx[3 * f[b] + q]];
+                    if (t >= 0 && t != last) {
+                        auto& lst = touch[t];
+                        if (lst.empty() || lst.back() != std::make_pair(k, int(r))) lst.emplace_back(k, int(r));
+                    }
+                }
+            }
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < nCoords; ++c) {
+        const int v = coords[c] / 3, ax = coords[c] % 3;
+        const auto& lst = touch[target[v]];
+        std::vector<double> vv(verts, verts + size_t(nVerts) * 3);
+        FrozenMesh mp{vv.data(), idx};
+        // central difference with step hs; sigma(+) - sigma(-) is formed as
+        // 4 pi [2 Re(conj(S) (dS+ - dS-)) + |dS+|^2 - |dS-|^2], which has
+        // no cancellation of |S|^2
+        auto central = [&](double hs) {
+            std::vector<double> dP(2 * nJobs, 0.0), dM(2 * nJobs, 0.0);
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                vv[3 * size_t(v) + ax] = verts[3 * size_t(v) + ax] + sgn * hs;
+                std::vector<double>& dS = sgn > 0 ? dP : dM;
+                for (const auto& kr : lst) {
+                    const int k = kr.first, r = kr.second;
+                    const int32_t* f = ids + idBase[k] + size_t(r) * J[k].maxB;
+                    const int cc = cnt[rayBase[k] + r];
+                    double a0, b0, a1, b1;
+                    frozen_ray(m, J[k], r, f, cc, a0, b0);
+                    frozen_ray(mp, J[k], r, f, cc, a1, b1);
+                    dS[2 * k] += a1 - a0;
+                    dS[2 * k + 1] += b1 - b0;
+                }
+            }
+            vv[3 * size_t(v) + ax] = verts[3 * size_t(v) + ax];
+            double sum = 0;
+            for (int k = 0; k < nJobs; ++k) {
+                if (w[k] == 0.0) continue;
+                const double xr = dP[2 * k] - dM[2 * k], xi = dP[2 * k + 1] - dM[2 * k + 1];
+                const double ds = 4.0 * M_PI *
+                                  (2.0 * (S[2 * k] * xr + S[2 * k + 1] * xi) +
+                                   (dP[2 * k] * dP[2 * k] + dP[2 * k + 1] * dP[2 * k + 1]) -
+                                   (dM[2 * k] * dM[2 * k] + dM[2 * k + 1] * dM[2 * k + 1]));
+                sum += w[k] * ds;
+            }
+            return sum / (2.0 * hs);
+        };
+        if (richardson) grad[c] = (4.0 * central(0.5 * h) - central(h)) / 3.0;
+        else grad[c] = central(h);
+    }
     return 0;
 }
 

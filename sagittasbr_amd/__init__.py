@@ -17,6 +17,7 @@ from ._native import (  # noqa: F401
     SbrError,
     SweepParams,
     angle_frame,
+    direction_jobs,
     header_functions,
     host_bvh,
     lib,
@@ -27,5 +28,5 @@ from ._native import (  # noqa: F401
 
 __all__ = [
     "SBR_BATCHED", "SBR_FP32", "SBR_FP64", "SBR_MIXED", "SBR_PER_ANGLE", "BvhOptions", "Context", "Job", "Plan",
-    "SbrError", "SweepParams", "angle_frame", "header_functions", "host_bvh", "lib", "load_model", "obj_parse_number", "sweep_jobs",
+    "SbrError", "SweepParams", "angle_frame", "direction_jobs", "header_functions", "host_bvh", "lib", "load_model", "obj_parse_number", "sweep_jobs",
 ]

@@ -164,6 +164,8 @@ def lib():
         "sbr_sweep_jobs": (C.c_int, [C.POINTER(SweepParams), C.c_int, C.c_int, _P, C.c_int]),
         "sbr_plan_create": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
         "sbr_plan_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+        "sbr_adjoint": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P, _P, _P]),
+        "sbr_mesh_destroy": (C.c_int, [_P, C.c_int]),
         "sbr_plan_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double),
                                         C.POINTER(C.c_int)]),
         "sbr_plan_run": (C.c_int, [_P]),
@@ -257,6 +259,19 @@ def sweep_jobs(params: SweepParams, mesh: int, precision=SBR_FP32):
     return jobs
 
 
+def direction_jobs(mesh: int, theta_deg, phi_deg, freq, grid_size, n_x, n_y, max_bounces, reflection_const=1.0):
+    """Jobs for explicit (theta, phi) directions in degrees (one job each)."""
+    th = np.atleast_1d(np.asarray(theta_deg, np.float64))
+    ph = np.atleast_1d(np.asarray(phi_deg, np.float64))
+    if th.shape != ph.shape:
+        raise ValueError("theta and phi must have the same length")
+    jobs = (Job * len(th))()
+    for k in range(len(th)):
+        jobs[k] = Job(mesh, max_bounces, n_x, n_y, float(freq), float(th[k]), float(ph[k]), float(grid_size),
+                      float(reflection_const))
+    return jobs
+
+
 class Plan:
     def __init__(self, ctx: "Context", jobs, precision=SBR_FP32, method=SBR_BATCHED):
         self.ctx = ctx
@@ -320,6 +335,7 @@ class Context:
 
     def __init__(self, device=0):
         self.h = _P()
+        self._mesh_nverts = {}
         _check(lib().sbr_ctx_create(device, C.byref(self.h)))
 
     def close(self):
@@ -341,7 +357,12 @@ class Context:
         mid = C.c_int()
         opts = C.byref(options) if options is not None else None
         _check(lib().sbr_mesh_create(self.h, _ptr(v), len(v), _ptr(f), len(f), scale, opts, C.byref(mid)), self.h)
+        self._mesh_nverts[mid.value] = len(v)
         return mid.value
+
+    def destroy_mesh(self, mesh):
+        _check(lib().sbr_mesh_destroy(self.h, mesh), self.h)
+        self._mesh_nverts.pop(mesh, None)
 
     def mesh_info(self, mesh, precision=SBR_FP32):
         mi = MeshInfo()
@@ -371,6 +392,26 @@ class Context:
         _check(lib().sbr_sweep(self.h, mesh, C.byref(params), precision, method, _ptr(reim), _ptr(sig), _ptr(db)),
                self.h)
         return reim[0::2] + 1j * reim[1::2], sig, db
+
+    def adjoint(self, mesh, jobs, dloss_dsigma=None):
+        """Frozen-path adjoint: returns (sigma[n_jobs], dloss_dxyz[n_verts, 3],
+        dsigma_dfreq[n_jobs]); dloss_dxyz is with respect to the coordinates
+        given to `mesh` (before its scale)."""
+        if not isinstance(jobs, C.Array):
+            arr = (Job * len(jobs))()
+            for i, j in enumerate(jobs):
+                arr[i] = j
+            jobs = arr
+        n = len(jobs)
+        nv = self._mesh_nverts[mesh]
+        w = np.zeros(n, np.float64) if dloss_dsigma is None else np.ascontiguousarray(dloss_dsigma, np.float64)
+        if w.shape != (n,):
+            raise ValueError("dloss_dsigma must have one value per job")
+        sig = np.zeros(n, np.float64)
+        dx = np.zeros(nv * 3, np.float64)
+        df = np.zeros(n, np.float64)
+        _check(lib().sbr_adjoint(self.h, mesh, C.cast(jobs, _P), n, _ptr(w), _ptr(sig), _ptr(dx), _ptr(df)), self.h)
+        return sig, dx.reshape(nv, 3), df
 
     def plan(self, jobs, precision=SBR_FP32, method=SBR_BATCHED) -> Plan:
         return Plan(self, jobs, precision, method)

@@ -31,6 +31,7 @@ hipError_t launch_retrace(const TraceArgs<float>& a, bool rec, bool cnt, int gri
 hipError_t launch_finalize(const Job<double>* jobs64, int n, const unsigned long long* fix, double* out,
                            hipStream_t s);
 size_t partial_bytes(bool fp64, int mode);
+hipError_t launch_adjoint(const AdjArgs& a, hipStream_t s);
 template <typename R> int trace_occupancy(int mode, int sd, bool pool);
 }  // namespace sbr
 
@@ -80,6 +81,7 @@ template <typename R> struct PrecMesh {
     Bvh<R> bvh;
     GpuBvh<R> gpu;
     DevBuf nodes, tris;
+    DevBuf tris64;  // FP32 tree: FP64 vertex records in tree order (80 B each)
     double buildMs = 0;
 };
 
@@ -91,6 +93,7 @@ struct MeshEntry {
     BuildOptions opt;
     PrecMesh<float> f32;
     PrecMesh<double> f64;
+    DevBuf adjVerts, adjIdx;  // adjoint: reference FP64 vertices (scaled) and indices, original order
 };
 
 }  // namespace
@@ -113,6 +116,7 @@ struct sbr_plan {
     std::vector<sbr_job> jobs;
     std::vector<int64_t> tileBase;
     std::vector<int> nTiles;
+    std::vector<int> meshIds;  // distinct meshes of the jobs
     int64_t totalTiles = 0;
     int maxDepth = 0;
     sbr_tuning tune{0, 0, 0, 0, 0};
@@ -158,6 +162,23 @@ template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
     HIP_TRY(&ctx->err, pm.tris.ensure(pm.gpu.tris.size()));
     HIP_TRY(&ctx->err, hipMemcpy(pm.nodes.p, pm.gpu.nodes.data(), pm.gpu.nodes.size(), hipMemcpyHostToDevice));
     HIP_TRY(&ctx->err, hipMemcpy(pm.tris.p, pm.gpu.tris.data(), pm.gpu.tris.size(), hipMemcpyHostToDevice));
+    if constexpr (std::is_same_v<R, float>) {
+        // reference FP64 vertices (rtRcsEntry.cu:136-144 with Real = double)
+        // of every triangle in this tree's order, for the mixed mode
+        const double sd = m.scale;
+        std::vector<double> t64(nt * 10);
+        for (size_t h = 0; h < nt; ++h) {
+            const size_t f = size_t(pm.bvh.perm[h]);
+            for (int c = 0; c < 3; ++c) {
+                uint32_t vi = m.idx[3 * f + c];
+                for (int k = 0; k < 3; ++k) t64[10 * h + 3 * c + k] = double(m.xyz[3 * size_t(vi) + k]) * sd;
+            }
+            long long fid = static_cast<long long>(f);
+            std::memcpy(&t64[10 * h + 9], &fid, sizeof(double));
+        }
+        HIP_TRY(&ctx->err, pm.tris64.ensure(t64.size() * sizeof(double)));
+        HIP_TRY(&ctx->err, hipMemcpy(pm.tris64.p, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     pm.built = true;
     pm.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (sizeof(R) == 4) ctx->tableDirty32 = true;
@@ -171,11 +192,16 @@ template <typename R> int upload_table(sbr_ctx* ctx) {
     if (!dirty && buf.p) return 0;
     std::vector<Mesh<R>> tab(ctx->meshes.size());
     for (size_t i = 0; i < ctx->meshes.size(); ++i) {
-        PrecMesh<R>& pm = prec_of<R>(*ctx->meshes[i]);
         Mesh<R> mm{};
+        if (!ctx->meshes[i]) {
+            tab[i] = mm;  // destroyed
+            continue;
+        }
+        PrecMesh<R>& pm = prec_of<R>(*ctx->meshes[i]);
         if (pm.built) {
             mm.nodes = pm.nodes.p;
             mm.tris = pm.tris.p;
+            mm.tris64 = pm.tris64.p;
             for (int k = 0; k < 3; ++k) {
                 mm.rootLo[k] = pm.gpu.rootLo[k];
                 mm.rootHi[k] = pm.gpu.rootHi[k];
@@ -302,8 +328,8 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
     const int ts = pl->tileSide;
     for (int i = 0; i < pl->nJobs; ++i) {
         const sbr_job& j = pl->jobs[i];
-        if (j.mesh < 0 || j.mesh >= int(ctx->meshes.size()))
-            return fail(&ctx->err, "job " + std::to_string(i) + ": bad mesh id");
+        if (j.mesh < 0 || j.mesh >= int(ctx->meshes.size()) || !ctx->meshes[j.mesh])
+            return fail(&ctx->err, "job " + std::to_string(i) + ": bad or destroyed mesh id");
         if (j.n_x <= 0 || j.n_y <= 0 || j.max_bounces < 0 || !(j.freq > 0) || !(j.grid_size > 0))
             return fail(&ctx->err, "job " + std::to_string(i) + ": bad grid, bounce count, frequency or size");
         if (int64_t(j.n_x) * j.n_y > 0x7fffffffll)
@@ -505,6 +531,8 @@ template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int
                                       double* rReIm) {
     sbr_ctx* ctx = pl->ctx;
     hipStream_t s = ctx->stream;
+    for (int id : pl->meshIds)
+        if (!ctx->meshes[id]) return fail(&ctx->err, "a mesh of this plan was destroyed");
     if (pl->tune.use_graph && !record && !count) {
         if (!pl->graphExec) {
             HIP_TRY(&ctx->err, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -567,6 +595,9 @@ int create_plan(sbr_ctx* ctx, const sbr_job* jobs, int n, int precision, int met
     pl->method = method;
     pl->nJobs = n;
     pl->tileSide = default_tile(method);
+    for (int i = 0; i < n; ++i) pl->meshIds.push_back(jobs[i].mesh);
+    std::sort(pl->meshIds.begin(), pl->meshIds.end());
+    pl->meshIds.erase(std::unique(pl->meshIds.begin(), pl->meshIds.end()), pl->meshIds.end());
     pl->jobs.assign(jobs, jobs + n);
     HIP_TRY(&ctx->err, hipEventCreate(&pl->ev0));
     HIP_TRY(&ctx->err, hipEventCreate(&pl->ev1));
@@ -652,6 +683,17 @@ int sbr_mesh_create(sbr_ctx* ctx, const float* xyz, int n_verts, const uint32_t*
     return 0;
 }
 
+int sbr_mesh_destroy(sbr_ctx* ctx, int mesh_id) {
+    if (!ctx) return fail(nullptr, "null context");
+    if (mesh_id < 0 || mesh_id >= int(ctx->meshes.size()) || !ctx->meshes[mesh_id])
+        return fail(&ctx->err, "bad or destroyed mesh id");
+    HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
+    HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
+    ctx->meshes[mesh_id].reset();
+    ctx->tableDirty32 = ctx->tableDirty64 = true;
+    return 0;
+}
+
 struct sbr_bvh {
     int precision = 0;
     Bvh<float> f;
@@ -729,7 +771,7 @@ int sbr_bvh_destroy(sbr_bvh* b) {
 }
 
 static MeshEntry* get_mesh(sbr_ctx* ctx, int id, int precision) {
-    if (id < 0 || id >= int(ctx->meshes.size())) return nullptr;
+    if (id < 0 || id >= int(ctx->meshes.size()) || !ctx->meshes[id]) return nullptr;
     MeshEntry* m = ctx->meshes[id].get();
     if (precision == SBR_FP64 ? ensure_built<double>(ctx, *m) : ensure_built<float>(ctx, *m)) return nullptr;
     return m;
@@ -1081,6 +1123,81 @@ int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int ang
     rc = run();
     sbr_plan_destroy(pl);
     return rc;
+}
+
+int sbr_adjoint(sbr_ctx* ctx, int mesh_id, const sbr_job* jobs, int n_jobs, const double* dloss_dsigma,
+                double* sigma, double* dloss_dxyz, double* dsigma_dfreq) {
+    if (!ctx) return fail(nullptr, "null context");
+    if (mesh_id < 0 || mesh_id >= int(ctx->meshes.size()) || !ctx->meshes[mesh_id])
+        return fail(&ctx->err, "bad or destroyed mesh id");
+    if (n_jobs < 0 || (n_jobs > 0 && !jobs)) return fail(&ctx->err, "bad job list");
+    for (int i = 0; i < n_jobs; ++i)
+        if (jobs[i].mesh != mesh_id) return fail(&ctx->err, "job " + std::to_string(i) + " is on another mesh");
+    HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
+    MeshEntry& m = *ctx->meshes[mesh_id];
+    const size_t nv = m.xyz.size() / 3;
+    if (!m.adjVerts.p) {
+        std::vector<double> v64(m.xyz.size());
+        for (size_t k = 0; k < v64.size(); ++k) v64[k] = double(m.xyz[k]) * m.scale;
+        HIP_TRY(&ctx->err, m.adjVerts.ensure(v64.size() * sizeof(double)));
+        HIP_TRY(&ctx->err, hipMemcpy(m.adjVerts.p, v64.data(), v64.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(&ctx->err, m.adjIdx.ensure(m.idx.size() * sizeof(uint32_t)));
+        HIP_TRY(&ctx->err,
+                hipMemcpy(m.adjIdx.p, m.idx.data(), m.idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    DevBuf dV, dF, dIds, dCnt, dL, dRe;
+    HIP_TRY(&ctx->err, dV.ensure(std::max<size_t>(nv * 3, 1) * sizeof(double)));
+    HIP_TRY(&ctx->err, dF.ensure(std::max(n_jobs, 1) * sizeof(double)));
+    HIP_TRY(&ctx->err, hipMemsetAsync(dV.p, 0, nv * 3 * sizeof(double), ctx->stream));
+    HIP_TRY(&ctx->err, hipMemsetAsync(dF.p, 0, std::max(n_jobs, 1) * sizeof(double), ctx->stream));
+    for (int i = 0; i < n_jobs; ++i) {
+        const sbr_job& jb = jobs[i];
+        // forward: mixed precision (FP64 outcome per ray), recording hit faces
+        sbr_plan* pl = nullptr;
+        if (create_plan(ctx, &jb, 1, SBR_MIXED, SBR_PER_ANGLE, true, &pl)) return 1;
+        std::unique_ptr<sbr_plan, int (*)(sbr_plan*)> guard(pl, sbr_plan_destroy);
+        const size_t nr = size_t(jb.n_x) * jb.n_y;
+        const size_t mb = std::max(jb.max_bounces, 1);
+        HIP_TRY(&ctx->err, dIds.ensure(nr * mb * sizeof(int32_t)));
+        HIP_TRY(&ctx->err, dCnt.ensure(nr * sizeof(int32_t)));
+        HIP_TRY(&ctx->err, dL.ensure(nr * sizeof(double)));
+        HIP_TRY(&ctx->err, dRe.ensure(nr * 2 * sizeof(double)));
+        HIP_TRY(&ctx->err, pl->dRecFlag.ensure(nr));
+        HIP_TRY(&ctx->err, hipMemsetAsync(dIds.p, 0xff, nr * mb * sizeof(int32_t), ctx->stream));
+        HIP_TRY(&ctx->err, hipMemsetAsync(dCnt.p, 0, nr * sizeof(int32_t), ctx->stream));
+        if (dispatch_launch(pl, true, false, static_cast<int32_t*>(dIds.p), static_cast<int32_t*>(dCnt.p),
+                            static_cast<double*>(dL.p), static_cast<double*>(dRe.p)))
+            return 1;
+        double reim[2], sg;
+        if (sbr_plan_results(pl, reim, &sg, nullptr)) return 1;
+        if (sigma) sigma[i] = sg;
+        // adjoint of this job's sigma along the recorded paths
+        AdjArgs a{};
+        make_frame<double>(jb.theta_deg, jb.phi_deg, jb.grid_size, a.D.llc, a.D.uv, a.D.vv, a.D.dir);
+        make_phys<double>(jb, a.D);
+        a.D.nx = jb.n_x;
+        a.D.ny = jb.n_y;
+        a.D.maxB = int(mb);
+        a.verts = static_cast<const double*>(m.adjVerts.p);
+        a.idx = static_cast<const unsigned*>(m.adjIdx.p);
+        a.ids = static_cast<const int32_t*>(dIds.p);
+        a.cnt = static_cast<const int32_t*>(dCnt.p);
+        a.Sre = reim[0];
+        a.Sim = reim[1];
+        a.w = dloss_dsigma ? dloss_dsigma[i] : 0.0;
+        a.freq = jb.freq;
+        a.scale = m.scale;
+        a.dvert = static_cast<double*>(dV.p);
+        a.dfreq = static_cast<double*>(dF.p) + i;
+        a.nRays = int(nr);
+        HIP_TRY(&ctx->err, launch_adjoint(a, ctx->stream));
+        HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
+    }
+    if (dloss_dxyz && nv)
+        HIP_TRY(&ctx->err, hipMemcpy(dloss_dxyz, dV.p, nv * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (dsigma_dfreq && n_jobs)
+        HIP_TRY(&ctx->err, hipMemcpy(dsigma_dfreq, dF.p, n_jobs * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
 }
 
 }  // extern "C"

@@ -35,6 +35,10 @@ template <typename R> struct alignas(16) Job {
 template <typename R> struct alignas(16) Mesh {
     const void* nodes;
     const void* tris;
+    // FP32 tree only: the reference FP64 vertices of each triangle, in the
+    // FP32 tree's order (v0 v1 v2 as 9 doubles + face id), for the mixed
+    // mode's FP64 recompute of a hit
+    const void* tris64;
     R rootLo[3], rootHi[3];
     int rootRef;
     int nTris;
@@ -75,6 +79,21 @@ template <typename R> struct TraceArgs {
     float errK;
     unsigned long long* jobFix;
     int8_t* recFlag;
+};
+
+// Arguments of the frozen-path adjoint kernel (one launch per job).
+struct AdjArgs {
+    Job<double> D;
+    const double* verts;  // n_verts * 3, reference FP64 vertices (scaled)
+    const unsigned* idx;  // n_tris * 3
+    const int32_t* ids;   // rays * maxB original face ids
+    const int32_t* cnt;   // rays
+    double Sre, Sim;      // the job's complex sum
+    double w;             // upstream d loss / d sigma of the job
+    double freq, scale;
+    double* dvert;        // n_verts * 3, d loss / d (input coordinate), accumulated
+    double* dfreq;        // d sigma / d f of the job (unweighted), accumulated
+    int nRays;
 };
 
 }  // namespace sbr

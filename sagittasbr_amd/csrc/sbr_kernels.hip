@@ -126,15 +126,16 @@ __device__ __forceinline__ TriRec<float> load_tri(const float* tris, int i) {
     V3<float> v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
     return {v0, sub(v1, v0), sub(v2, v0), __float_as_int(c.y), c.z, c.w};
 }
-// Exact FP64 vertices of an FP32 record (for the mixed mode's recompute).
-__device__ __forceinline__ void load_tri_verts64(const float* tris, int i, V3<double>& v0, V3<double>& e1,
+// Reference FP64 vertices of FP32-tree triangle i (mixed mode's recompute);
+// the edges are formed as rtKernels.cu:13-14 forms them, in FP64.
+__device__ __forceinline__ void load_tri_verts64(const void* tris64, int i, V3<double>& v0, V3<double>& e1,
                                                  V3<double>& e2, int& face) {
-    gptr<vf4> p = as_global<vf4>(tris) + 3 * i;
-    vf4 a = p[0], b = p[1], c = p[2];
-    v0 = {double(a.x), double(a.y), double(a.z)};
-    e1 = {double(a.w) - v0.x, double(b.x) - v0.y, double(b.y) - v0.z};
-    e2 = {double(b.z) - v0.x, double(b.w) - v0.y, double(c.x) - v0.z};
-    face = __float_as_int(c.y);
+    gptr<vd2> p = as_global<vd2>(tris64) + 5 * i;
+    vd2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    v0 = {a.x, a.y, b.x};
+    e1 = {b.y - v0.x, c.x - v0.y, c.y - v0.z};
+    e2 = {d.x - v0.x, d.y - v0.y, e.x - v0.z};
+    face = static_cast<int>(__double_as_longlong(e.y));
 }
 __device__ __forceinline__ TriRec<double> load_tri(const double* tris, int i) {
     gptr<vd2> p = as_global<vd2>(tris) + 5 * i;
@@ -643,7 +644,7 @@ __device__ __forceinline__ int ray_step_mixed(const Job<float>& J, const Mesh<fl
     // FP64 recompute of the hit on the chosen triangle (rtKernels.cu:13-41)
     V3<double> v0, e1, e2;
     int face;
-    load_tri_verts64(static_cast<const float*>(M.tris), h, v0, e1, e2, face);
+    load_tri_verts64(M.tris64, h, v0, e1, e2, face);
     V3<double> p = cross3(d, e2);
     double det = dot3(e1, p);
     double invd = 1.0 / det;
@@ -1028,6 +1029,157 @@ __global__ void zero_words(unsigned* p, size_t n) {
     if (i < n) p[i] = 0u;
 }
 
+// ------------------------------------------------------------- adjoint
+// Frozen-path adjoint of sigma (no counterpart in the reference). For each
+// ray the recorded hit faces fix the path; the bounce geometry is recomputed
+// in FP64 exactly as the mixed/FP64 ray computes it (rtKernels.cu:13-41,
+// 144-152) and differentiated in reverse:
+//   sigma = 4 pi |S|^2, S = sum_r c_r, c_r = mag_r (-sin ph_r, cos ph_r),
+//   ph_r = 2 k L_r (mod 2 pi), mag_r = pre cos(theta_r) R^n,
+//   L_r = sum_b t_b, t_b = ((v0 - o_b).c_b) / (d_b.c_b) (the Moller-Trumbore
+//   distance in plane form), c_b = e1 x e2, n_b = c_b / |c_b|,
+//   d_{b+1} = w/|w|, w = d_b - 2 (d_b.n_b) n_b, o_{b+1} = p_b + 1e-4 n_b.
+// d sigma / d f follows from k = 2 pi f / c0 and pre ~ k (poKernels.cu:98-101).
+
+struct AdjBounce {
+    V3<double> v0, e1, e2, c, n, p, dn;
+    double t, cl, s, wl;
+};
+
+__device__ __forceinline__ V3<double> vtx64(const AdjArgs& a, int f, int k) {
+    const unsigned vi = a.idx[3 * f + k];
+    return {a.verts[3 * size_t(vi)], a.verts[3 * size_t(vi) + 1], a.verts[3 * size_t(vi) + 2]};
+}
+
+// Geometry of one bounce on face f from (o, d), with the forward's operations.
+__device__ __forceinline__ AdjBounce adj_bounce(const AdjArgs& a, int f, V3<double> o, V3<double> d) {
+    AdjBounce b;
+    b.v0 = vtx64(a, f, 0);
+    b.e1 = sub(vtx64(a, f, 1), b.v0);
+    b.e2 = sub(vtx64(a, f, 2), b.v0);
+    V3<double> pv = cross3(d, b.e2);
+    double det = dot3(b.e1, pv);
+    double invd = 1.0 / det;
+    V3<double> q = cross3(sub(o, b.v0), b.e1);
+    b.t = dot3(b.e2, q) * invd;
+    b.c = cross3(b.e1, b.e2);
+    b.cl = sqrt(dot3(b.c, b.c));
+    b.n = unit3(b.c);
+    b.p = axpy(b.t, d, o);
+    b.s = dot3(d, b.n);
+    V3<double> wv = axpy(-(2.0 * b.s), b.n, d);
+    b.wl = sqrt(dot3(wv, wv));
+    b.dn = unit3(wv);
+    return b;
+}
+
+__device__ __forceinline__ void adj_add(double* p, V3<double> g, double s) {
+    atomicAdd(p, g.x * s);
+    atomicAdd(p + 1, g.y * s);
+    atomicAdd(p + 2, g.z * s);
+}
+
+__global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
+    __shared__ double red[4];
+    const int ray = blockIdx.x * 256 + threadIdx.x;
+    double dfr = 0.0;
+    const int B = ray < a.nRays ? a.cnt[ray] : 0;
+    if (B > 0) {
+        const Job<double>& D = a.D;
+        const int32_t* fids = a.ids + size_t(ray) * D.maxB;
+        const int i = ray % D.nx, j = ray / D.nx;
+        const V3<double> o0 = ray_origin(D, i, j);
+        const V3<double> d0{D.dir[0], D.dir[1], D.dir[2]};
+        // forward: path length and the last hit's cos(theta)
+        V3<double> o = o0, d = d0;
+        double L = 0.0, cosT = 0.0;
+        for (int b = 0; b < B; ++b) {
+            AdjBounce g = adj_bounce(a, fids[b], o, d);
+            cosT = dot3(g.n, V3<double>{-d.x, -d.y, -d.z});
+            L += g.t;
+            d = g.dn;
+            o = axpy(1e-4, g.n, g.p);
+        }
+        if (cosT > 0.0) {
+            double refl = 1.0;
+            for (int b = 0; b < B; ++b) refl *= D.refl;
+            const double ph = fmod((2.0 * D.kd) * L, D.twoPid);
+            const double mag = (D.pre * cosT) * refl;
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            const double cre = -mag * sn, cim = mag * cs;
+            // d sigma / d (Re c, Im c) = 8 pi (Re S, Im S)
+            const double gre = 8.0 * M_PI * a.Sre, gim = 8.0 * M_PI * a.Sim;
+            const double rot = gre * (-cim) + gim * cre;  // g . (i c)
+            const double par = gre * cre + gim * cim;     // g . c
+            dfr = (par + 2.0 * L * D.kd * rot) / a.freq;
+            const double gL = a.w * 2.0 * D.kd * rot;
+            const double gC = a.w * par / cosT;
+            if (a.w != 0.0) {
+                V3<double> ob{0, 0, 0}, db{0, 0, 0};  // adjoints of o_{b+1}, d_{b+1}
+                for (int b = B - 1; b >= 0; --b) {
+                    // o_b, d_b: forward again from the ray start
+                    V3<double> ob_ = o0, db_ = d0;
+                    for (int k = 0; k < b; ++k) {
+                        AdjBounce g = adj_bounce(a, fids[k], ob_, db_);
+                        db_ = g.dn;
+                        ob_ = axpy(1e-4, g.n, g.p);
+                    }
+                    const AdjBounce g = adj_bounce(a, fids[b], ob_, db_);
+                    // o_{b+1} = p + 1e-4 n
+                    V3<double> pbar = ob;
+                    V3<double> nbar = axpy(1e-4, ob, V3<double>{0, 0, 0});
+                    // d_{b+1} = w / |w|
+                    const double pr = dot3(db, g.dn);
+                    V3<double> wbar{(db.x - pr * g.dn.x) / g.wl, (db.y - pr * g.dn.y) / g.wl,
+                                    (db.z - pr * g.dn.z) / g.wl};
+                    // w = d - 2 s n
+                    V3<double> dbar = wbar;
+                    const double sbar = -2.0 * dot3(g.n, wbar);
+                    nbar = axpy(-2.0 * g.s, wbar, nbar);
+                    // s = d . n
+                    dbar = axpy(sbar, g.n, dbar);
+                    nbar = axpy(sbar, db_, nbar);
+                    // cos(theta) = -n . d at the last hit
+                    if (b == B - 1) {
+                        nbar = axpy(-gC, db_, nbar);
+                        dbar = axpy(-gC, g.n, dbar);
+                    }
+                    // p = o + t d
+                    V3<double> obar = pbar;
+                    const double tbar = gL + dot3(pbar, db_);
+                    dbar = axpy(g.t, pbar, dbar);
+                    // t = ((v0 - o).c) / (d.c)
+                    const double den = dot3(db_, g.c);
+                    const double tr = tbar / den;
+                    obar = axpy(-tr, g.c, obar);
+                    dbar = axpy(-tr * g.t, g.c, dbar);
+                    V3<double> v0bar = axpy(tr, g.c, V3<double>{0, 0, 0});
+                    V3<double> cbar = axpy(tr, sub(g.v0, g.p), V3<double>{0, 0, 0});
+                    // n = c / |c|
+                    const double nn = dot3(nbar, g.n);
+                    cbar = axpy(1.0 / g.cl, V3<double>{nbar.x - nn * g.n.x, nbar.y - nn * g.n.y, nbar.z - nn * g.n.z},
+                                cbar);
+                    // c = e1 x e2
+                    const V3<double> e1bar = cross3(g.e2, cbar), e2bar = cross3(cbar, g.e1);
+                    v0bar = sub(v0bar, V3<double>{e1bar.x + e2bar.x, e1bar.y + e2bar.y, e1bar.z + e2bar.z});
+                    const int f = fids[b];
+                    adj_add(a.dvert + 3 * size_t(a.idx[3 * f]), v0bar, a.scale);
+                    adj_add(a.dvert + 3 * size_t(a.idx[3 * f + 1]), e1bar, a.scale);
+                    adj_add(a.dvert + 3 * size_t(a.idx[3 * f + 2]), e2bar, a.scale);
+                    ob = obar;
+                    db = dbar;
+                }
+            }
+        }
+    }
+    // d sigma / d f: wave sums, then one atomic per workgroup
+    dfr = wave_sum(dfr);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dfr;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(a.dfreq, red[0] + red[1] + red[2] + red[3]);
+}
+
 template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL>
 hipError_t launch_one(const TraceArgs<R>& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST, POOL>), dim3(grid), dim3(kWgSize), 0, s, a);
@@ -1076,6 +1228,12 @@ hipError_t launch_retrace(const TraceArgs<float>& a, bool rec, bool cnt, int gri
     if (rec) hipLaunchKernelGGL((retrace_kernel<true, false>), dim3(grid), dim3(kWgSize), 0, s, a);
     else if (cnt) hipLaunchKernelGGL((retrace_kernel<false, true>), dim3(grid), dim3(kWgSize), 0, s, a);
     else hipLaunchKernelGGL((retrace_kernel<false, false>), dim3(grid), dim3(kWgSize), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint(const AdjArgs& a, hipStream_t s) {
+    if (a.nRays <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adjoint_kernel, dim3((a.nRays + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -49,6 +49,11 @@ def lib():
         L.orc_sweep.argtypes = [_P, _dp, _ip, C.c_int, _ip, C.c_int, _dp, _lp]
         L.orc_sweep.restype = C.c_int
         L.orc_num_threads.restype = C.c_int
+        L.orc_frozen_sum.argtypes = [_dp, _up, _dp, _ip, _ip, _ip, _dp]
+        L.orc_frozen_sum.restype = C.c_int
+        L.orc_frozen_fd.argtypes = [_dp, C.c_int, _up, C.c_int, C.c_int, _dp, _ip, _ip, _ip, _dp, C.c_int, _ip,
+                                    C.c_double, C.c_int, _dp, _dp]
+        L.orc_frozen_fd.restype = C.c_int
         _lib = L
     return _lib
 
@@ -228,3 +233,51 @@ def canonical_tree(bounds, links, root):
             cl[i, 0] = pos[l]
             cl[i, 1] = pos[r]
     return cb, cl
+
+
+# ------------------------------------------------------------ frozen paths
+def job_arrays(theta_deg, phi_deg, grid_size, freq, n_x, n_y, max_bounces, reflection_const=1.0):
+    """(pj, ij) arrays of the frozen-path functions for explicit directions."""
+    th = np.atleast_1d(np.asarray(theta_deg, np.float64))
+    ph = np.atleast_1d(np.asarray(phi_deg, np.float64))
+    fr = np.broadcast_to(np.asarray(freq, np.float64), th.shape)
+    pj = np.stack([th, ph, np.full_like(th, grid_size), fr, np.full_like(th, reflection_const)], 1)
+    ij = np.tile(np.array([n_x, n_y, max_bounces], np.int32), (len(th), 1))
+    return np.ascontiguousarray(pj), np.ascontiguousarray(ij)
+
+
+def direction_params(theta, phi, grid_size, freq, n_x, n_y, max_bounces, reflection_const=1.0):
+    """A one-angle sweep whose frame is the frame of direction (theta, phi)."""
+    return Params(theta_start=float(theta), theta_end=float(theta), theta_samples=1, phi_start=float(phi),
+                  phi_end=float(phi), phi_samples=1, grid_size=grid_size, n_x=n_x, n_y=n_y,
+                  max_bounces=max_bounces, reflection_const=reflection_const, freq=freq)
+
+
+def frozen_sum(verts64, faces, pj, ij, ids, cnt):
+    out = np.zeros(2, np.float64)
+    rc = lib().orc_frozen_sum(np.ascontiguousarray(verts64, np.float64).reshape(-1),
+                              np.ascontiguousarray(faces, np.uint32).reshape(-1), np.ascontiguousarray(pj, np.float64),
+                              np.ascontiguousarray(ij, np.int32), np.ascontiguousarray(ids, np.int32).reshape(-1),
+                              np.ascontiguousarray(cnt, np.int32).reshape(-1), out)
+    assert rc == 0
+    return complex(out[0], out[1])
+
+
+def frozen_fd(verts64, faces, pj, ij, ids, cnt, w, coords, h, richardson=False):
+    """Central differences of sum_j w_j sigma_j along frozen paths for
+    coordinates coords (3*vertex + axis); with richardson, the O(h^4)
+    combination of steps h and h/2. Returns (grad, sigma)."""
+    v = np.ascontiguousarray(verts64, np.float64).reshape(-1)
+    f = np.ascontiguousarray(faces, np.uint32).reshape(-1)
+    pj = np.ascontiguousarray(pj, np.float64)
+    ij = np.ascontiguousarray(ij, np.int32)
+    nj = len(pj)
+    coords = np.ascontiguousarray(coords, np.int32)
+    g = np.zeros(len(coords), np.float64)
+    sg = np.zeros(nj, np.float64)
+    rc = lib().orc_frozen_fd(v, len(v) // 3, f, len(f) // 3, nj, pj.reshape(-1), ij.reshape(-1),
+                             np.ascontiguousarray(ids, np.int32).reshape(-1),
+                             np.ascontiguousarray(cnt, np.int32).reshape(-1), np.ascontiguousarray(w, np.float64),
+                             len(coords), coords, float(h), int(richardson), g, sg)
+    assert rc == 0, rc
+    return g, sg
