@@ -57,6 +57,10 @@ def parse():
                     help="only the timed batched sweeps (for counter collection under rocprofv3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (rocprof FETCH/WRITE_SIZE calibration)")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
+                    help="c2 (default) is the metric's configuration; c1, c3, c4 are the other configurations")
+    ap.add_argument("--stride", type=int, default=0,
+                    help="c1/c3/c4: every stride-th angle for the FP64 and per-angle comparisons (0 = default)")
     return ap.parse_args()
 
 
@@ -176,6 +180,91 @@ def timed(plan, steps, warmup):
     return 1000.0 * (time.perf_counter() - t0) / steps, kernel_ms
 
 
+def other_workload(args):
+    """c1 / c3 / c4: batched mixed sweep (autotuned, one persistent launch)
+    against the plain per-angle FP32 method and the FP64 path, the last two
+    on every stride-th angle (rates are per angle, accuracy per angle)."""
+    import torch  # noqa: F401  (device plumbing only)
+
+    import sagittasbr_amd as S
+    from sagittasbr_amd import workloads as W
+
+    t0 = time.perf_counter()
+    wl = W.WORKLOADS[args.workload]()
+    stride = args.stride or {"c1": 1, "c3": 31, "c4": 8}[args.workload]
+    log(f"{wl['desc']}; generated in {time.perf_counter() - t0:.1f} s")
+    ctx = S.Context(0)
+    t0 = time.perf_counter()
+    mids = [ctx.mesh(v, f) for v, f in wl["meshes"]]
+    build_s = time.perf_counter() - t0
+    log(f"{len(mids)} mesh(es) uploaded, FP32 BVH built in {build_s:.1f} s")
+
+    def jobs(prec, step=1):
+        out = []
+        for mi, kw in wl["sweeps"]:
+            js = S.sweep_jobs(S.SweepParams.make(**kw), mids[mi], prec)
+            out.extend(js[k] for k in range(len(js)))
+        out = out[::step]
+        return (S.Job * len(out))(*out)
+
+    jm = jobs(S.SBR_MIXED)
+    n = len(jm)
+    rays = sum(j.n_x * j.n_y for j in jm)
+    plan = ctx.plan(jm, S.SBR_MIXED, S.SBR_BATCHED)
+    log(f"{n} jobs, {rays:.3e} launched rays; autotuning")
+    tuning = plan.autotune(sample_jobs=min(n, 96 if args.workload == "c3" else 512), repeats=1)
+    if args.workload == "c4":
+        plan.tune(tuning["wg_per_cu"], tuning["lds_stack"], tuning["tile_rays"], 1, tuning["ray_pool"])
+        tuning["use_graph"] = 1
+    log(f"tuning {tuning}")
+    steps = args.steps if args.steps else 1
+    ms, kms = timed(plan, steps, args.warmup)
+    log(f"batched mixed {ms:.1f} ms/step")
+    z, sig, db = plan.results()
+    own = plan.count()
+    plan.close()
+
+    j32 = jobs(S.SBR_FP32, stride)
+    pa = ctx.plan(j32, S.SBR_FP32, S.SBR_PER_ANGLE)
+    t_pa = pa.autotune(sample_jobs=min(len(j32), 64), repeats=1)
+    pa_ms, _ = timed(pa, 1, 0)
+    pa.close()
+    per_angle_rate = len(j32) / (pa_ms / 1000.0)
+    log(f"per-angle FP32 on {len(j32)} angles: {pa_ms:.1f} ms")
+
+    j64 = jobs(S.SBR_FP64, stride)
+    p64 = ctx.plan(j64, S.SBR_FP64, S.SBR_BATCHED)
+    fp64_ms, _ = timed(p64, 1, 0)
+    z64, s64, db64 = p64.results()
+    cnt = p64.count()
+    p64.close()
+    log(f"FP64 on {len(j64)} angles: {fp64_ms:.1f} ms")
+    derr = np.abs(db[::stride] - db64)
+    strong = db64 > db64.max() - 30.0
+    scale = n / len(j64)
+    value = n / (ms / 1000.0)
+    return {
+        "metric": "angles/s (batched mixed-precision sweep, <=0.1 dB vs FP64), 1 GPU",
+        "value": round(value, 2), "unit": "angles/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic (procedural meshes)",
+        "config": {"workload": wl["desc"], "method": "batched", "precision": "mixed", "tuning": tuning,
+                   "bvh_build_s": round(build_s, 2)},
+        "ray_bounces_per_s_est": round(cnt["ray_bounces"] * scale / (ms / 1000.0), 1),
+        "launched_rays": rays,
+        "per_angle_fp32_angles_per_s": round(per_angle_rate, 2), "per_angle_fp32_tuning": t_pa,
+        "per_angle_sample": f"every {stride}th angle ({len(j32)})",
+        "batched_over_per_angle": round(value / per_angle_rate, 3),
+        "fp64_angles_per_s": round(len(j64) / (fp64_ms / 1000.0), 2),
+        "accuracy_vs_fp64_db": {"angles_compared": len(j64), "max": float(derr.max()),
+                                "median": float(np.median(derr)),
+                                "max_within_30dB_of_peak": float(derr[strong].max()),
+                                "frac_angles_le_0.1dB": float(np.mean(derr <= 0.1))},
+        "retraced_rays": own["retraced"],
+        "counters_fp64_sample": cnt,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -184,6 +273,11 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload != "c2":
+        if args.profile_run:
+            raise SystemExit("--profile-run applies to the c2 workload")
+        print(json.dumps(other_workload(args)))
         return
     import torch  # device plumbing only: the sweep runs in libsbr.so
 

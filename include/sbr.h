@@ -116,6 +116,9 @@ int sbr_abi_version(void);
 /* Replaces the device setup and buffer allocation of runRcsApp
  * (src/app/rtRcsEntry.cu:174-196, src/app/deviceBuffers.cu:12-29). */
 int sbr_ctx_create(int device, sbr_ctx** out);
+/* Frees the context's device data. Plans created on it and not yet destroyed
+ * lose their device data too: every later call on them fails with an error,
+ * and sbr_plan_destroy still frees their host side. */
 int sbr_ctx_destroy(sbr_ctx* ctx);
 /* Number of compute units of the context's device. */
 int sbr_ctx_cu_count(const sbr_ctx* ctx, int* out);

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <set>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -63,6 +64,11 @@ struct DevBuf {
     ~DevBuf() {
         if (p) (void)hipFree(p);
     }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
     hipError_t ensure(size_t n) {
         if (n <= bytes && p) return hipSuccess;
         if (p) {
@@ -106,6 +112,7 @@ struct sbr_ctx {
     std::vector<std::unique_ptr<MeshEntry>> meshes;
     DevBuf meshTable32, meshTable64;
     bool tableDirty32 = true, tableDirty64 = true;
+    std::set<sbr_plan*> plans;  // live plans, orphaned when the context goes
 };
 
 struct sbr_plan {
@@ -452,6 +459,19 @@ void drop_graph(sbr_plan* pl) {
     pl->graph = nullptr;
 }
 
+// Free a plan's device resources (graph, events, buffers); its host side
+// stays valid for sbr_plan_destroy.
+void release_plan_device(sbr_plan* pl) {
+    drop_graph(pl);
+    if (pl->ev0) (void)hipEventDestroy(pl->ev0);
+    if (pl->ev1) (void)hipEventDestroy(pl->ev1);
+    pl->ev0 = pl->ev1 = nullptr;
+    for (DevBuf* b : {&pl->dJobs, &pl->dPartials, &pl->dJobOut, &pl->dJobDone, &pl->dCounter, &pl->dCounters,
+                      &pl->dJobs64, &pl->dQueue, &pl->dQueueCount, &pl->dJobFix, &pl->dRecFlag})
+        b->release();
+    pl->ctx = nullptr;
+}
+
 // Enqueue one run: clear outputs/counters with fill kernels (no memset
 // nodes, so the sequence is safe to capture), then the trace launch(es).
 template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, int32_t* rIds, int32_t* rCnt, double* rL,
@@ -606,6 +626,7 @@ int create_plan(sbr_ctx* ctx, const sbr_job* jobs, int n, int precision, int met
         (void)hipEventDestroy(pl->ev1);
         return 1;
     }
+    ctx->plans.insert(pl.get());
     *out = pl.release();
     return 0;
 }
@@ -644,6 +665,9 @@ int sbr_ctx_destroy(sbr_ctx* ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    // plans still alive keep only their host side; every call on them fails
+    for (sbr_plan* pl : ctx->plans) release_plan_device(pl);
+    ctx->plans.clear();
     ctx->meshes.clear();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -925,6 +949,7 @@ int sbr_plan_create(sbr_ctx* ctx, const sbr_job* jobs, int n_jobs, int precision
 
 int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
     if (!pl || !t) return fail(nullptr, "null argument");
+    if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     if (t->lds_stack < 0 || t->wg_per_cu < 0) return fail(&pl->ctx->err, "negative tuning value");
     if (t->ray_pool < 0 || t->ray_pool > 2) return fail(&pl->ctx->err, "ray_pool must be 0, 1 or 2");
     if (t->tile_rays != 0 && t->tile_rays != 16 && t->tile_rays != 32 && t->tile_rays != 64)
@@ -941,18 +966,21 @@ int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
 
 int sbr_plan_run(sbr_plan* pl) {
     if (!pl) return fail(nullptr, "null plan");
+    if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     HIP_TRY(&pl->ctx->err, hipSetDevice(pl->ctx->device));
     return dispatch_launch(pl, false, false);
 }
 
 int sbr_plan_wait(sbr_plan* pl) {
     if (!pl) return fail(nullptr, "null plan");
+    if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     HIP_TRY(&pl->ctx->err, hipStreamSynchronize(pl->ctx->stream));
     return 0;
 }
 
 int sbr_plan_last_ms(sbr_plan* pl, double* ms) {
     if (!pl || !ms) return fail(nullptr, "null argument");
+    if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     if (!pl->ran) return fail(&pl->ctx->err, "plan has not run");
     float f = 0;
     HIP_TRY(&pl->ctx->err, hipEventSynchronize(pl->ev1));
@@ -963,6 +991,7 @@ int sbr_plan_last_ms(sbr_plan* pl, double* ms) {
 
 int sbr_plan_results(sbr_plan* pl, double* reim, double* sigma, double* dbsm) {
     if (!pl) return fail(nullptr, "null plan");
+    if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     if (!pl->ran) return fail(&pl->ctx->err, "plan has not run");
     sbr_ctx* ctx = pl->ctx;
     pl->out.assign(size_t(2) * pl->nJobs, 0.0);
@@ -986,6 +1015,7 @@ int sbr_plan_results(sbr_plan* pl, double* reim, double* sigma, double* dbsm) {
 
 int sbr_plan_count(sbr_plan* pl, sbr_counters* out) {
     if (!pl || !out) return fail(nullptr, "null argument");
+    if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     sbr_ctx* ctx = pl->ctx;
     HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
     if (dispatch_launch(pl, false, true)) return 1;
@@ -1005,6 +1035,7 @@ int sbr_plan_count(sbr_plan* pl, sbr_counters* out) {
 
 int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* best, double* best_ms, int* n_tried) {
     if (!pl) return fail(nullptr, "null plan");
+    if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     sbr_ctx* ctx = pl->ctx;
     if (repeats < 1) return fail(&ctx->err, "repeats must be >= 1");
     HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
@@ -1059,11 +1090,12 @@ int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* be
 
 int sbr_plan_destroy(sbr_plan* pl) {
     if (!pl) return 0;
-    (void)hipSetDevice(pl->ctx->device);
-    (void)hipStreamSynchronize(pl->ctx->stream);
-    drop_graph(pl);
-    if (pl->ev0) (void)hipEventDestroy(pl->ev0);
-    if (pl->ev1) (void)hipEventDestroy(pl->ev1);
+    if (pl->ctx) {
+        (void)hipSetDevice(pl->ctx->device);
+        (void)hipStreamSynchronize(pl->ctx->stream);
+        pl->ctx->plans.erase(pl);
+        release_plan_device(pl);
+    }
     delete pl;
     return 0;
 }

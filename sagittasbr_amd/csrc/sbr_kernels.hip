@@ -328,7 +328,9 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     int cur = m.rootRef;
     int sp = 0;
     for (;;) {
-        const float lim = closest + tErr;
+        // the true closest distance may lie up to tBestErr beyond the FP32
+        // one; a box is pruned only when it starts beyond that as well
+        const float lim = closest + tBestErr + tErr;
         if (cur >= 0) {
             NodeRec<float> n = load_node(nodes, cur);
             if (CNT) w.visits += 2;
@@ -386,13 +388,15 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
                 bool ambig = (fabsf(adet - 1.0e-6f) <= ddet) || nearEdge ||
                              ((inside || nearEdge) && fabsf(t - 0.001f) <= terr);
                 if (ambig && t >= 0.001f - terr) tAmb = fminf(tAmb, t - terr);
+                // t2: lowest possible true distance of any other hit (each
+                // competitor with its own error bound)
                 if (inside && t >= 0.001f && t <= closest) {
-                    if (closest < 1e19f) t2 = fminf(t2, closest);
+                    if (closest < 1e19f) t2 = fminf(t2, closest - tBestErr);
                     closest = t;
                     tBestErr = terr;
                     hit = first + k;
                 } else if (inside && t > closest) {
-                    t2 = fminf(t2, t);
+                    t2 = fminf(t2, t - terr);
                 }
             }
         }
@@ -407,7 +411,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
             } else {
                 Ov<float>::unpack(ov[sp - SD], ee, rr);
             }
-            if (ee <= closest + tErr) {
+            if (ee <= closest + tBestErr + tErr) {
                 cur = rr;
                 found = true;
                 break;
@@ -420,7 +424,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     } else {
         float tol = 2.0f * tBestErr + 1.0e-6f * closest;
         if (tAmb <= closest + tol) amb = true;
-        if (t2 - closest <= tol) amb = true;
+        if (t2 <= closest + tol) amb = true;
     }
     tHit = closest;
     return hit;

@@ -281,3 +281,83 @@ def frozen_fd(verts64, faces, pj, ij, ids, cnt, w, coords, h, richardson=False):
                              len(coords), coords, float(h), int(richardson), g, sg)
     assert rc == 0, rc
     return g, sg
+
+
+# ------------------------------------------- reference HIP kernels (GPU)
+REF_GPU_SO = {
+    "f32": os.path.join(ROOT, "oracle", "_ref", "libref_gpu_f32.so"),
+    "f64": os.path.join(ROOT, "oracle", "_ref", "libref_gpu_f64.so"),
+}
+_ref_gpu = {}
+
+
+def ref_gpu_lib(prec="f64"):
+    """The reference's own rtKernels.cu / poKernels.cu / bvhBuilder.cpp built
+    from its sources by oracle/Makefile, with oracle/ref_gpu_harness.hip as
+    the launcher; None when not built."""
+    if prec not in _ref_gpu:
+        if not os.path.exists(REF_GPU_SO[prec]):
+            return None
+        L = C.CDLL(REF_GPU_SO[prec])
+        L.refgpu_real_size.restype = C.c_int
+        L.refgpu_create.restype = _P
+        L.refgpu_create.argtypes = [_fp, C.c_int, _up, C.c_int, C.c_double]
+        L.refgpu_destroy.argtypes = [_P]
+        L.refgpu_sweep.argtypes = [_P, _dp, _ip, _ip, C.c_int, C.c_int, _dp, _dp]
+        L.refgpu_sweep.restype = C.c_int
+        L.refgpu_rays.argtypes = [_P, _dp, _ip, C.c_int, _ip, _dp]
+        L.refgpu_rays.restype = C.c_int
+        _ref_gpu[prec] = L
+    return _ref_gpu[prec]
+
+
+class RefGpu:
+    """Reference HIP kernels on one mesh (reference BVH, reference launch)."""
+
+    def __init__(self, verts, faces, prec="f64", scale=1.0):
+        self.L = ref_gpu_lib(prec)
+        if self.L is None:
+            raise RuntimeError(f"reference GPU library not built: {REF_GPU_SO[prec]}")
+        v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+        f = np.ascontiguousarray(faces, np.uint32).reshape(-1)
+        self.h = self.L.refgpu_create(v, len(v) // 3, f, len(f) // 3, float(scale))
+        if not self.h:
+            raise RuntimeError("refgpu_create failed")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.refgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def _pp(p: Params):
+        pd = np.array([p.theta_start, p.theta_end, p.phi_start, p.phi_end, p.grid_size, p.freq, p.reflection_const],
+                      np.float64)
+        pi = np.array([p.theta_samples, p.phi_samples, p.n_x, p.n_y, p.max_bounces], np.int32)
+        return pd, pi
+
+    def sweep(self, p: Params, angles=None, host_sum=False):
+        """(complex sum per angle, (rt_ms, po_ms) kernel totals). host_sum:
+        the sums are the reference PO term of the RT kernel's per-ray outputs
+        added on the host, without the PO kernel's block reduction."""
+        a = np.arange(p.n_angles, dtype=np.int32) if angles is None else np.ascontiguousarray(angles, np.int32)
+        pd, pi = self._pp(p)
+        reim = np.zeros(2 * len(a), np.float64)
+        t = np.zeros(2, np.float64)
+        rc = self.L.refgpu_sweep(self.h, pd, pi, a, len(a), int(host_sum), reim, t)
+        if rc:
+            raise RuntimeError(f"refgpu_sweep failed ({rc})")
+        return reim[0::2] + 1j * reim[1::2], (float(t[0]), float(t[1]))
+
+    def rays(self, p: Params, angle):
+        pd, pi = self._pp(p)
+        n = p.n_x * p.n_y
+        cnt = np.zeros(n, np.int32)
+        dist = np.zeros(n, np.float64)
+        rc = self.L.refgpu_rays(self.h, pd, pi, int(angle), cnt, dist)
+        if rc:
+            raise RuntimeError(f"refgpu_rays failed ({rc})")
+        return cnt, dist

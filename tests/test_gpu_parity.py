@@ -273,3 +273,43 @@ def test_autotune(ctx, tri_duct):
         z, _, _ = pl.results()
         assert np.array_equal(z, z0)
         pl.close()
+
+
+@pytest.mark.parametrize("theta,phi", [(92.0, 223.0), (88.0, 230.0), (90.0, 242.0), (60.0, 217.0), (100.0, 240.0)])
+def test_mixed_airframe_fin_angles(ctx, theta, phi):
+    """Regression: on the airframe's fin two triangles are hit 8e-5 m apart
+    and the farther one has a tiny determinant (distance error ~5e-3 m).
+    The mixed mode must flag such a ray (the competitor's own error bound
+    counts) and end with the FP64 outcome: per-ray bit-exact against the
+    FP64 path (itself bit-exact with the FP64 oracle)."""
+    from sagittasbr_amd import workloads as W
+
+    wl = W.c3((3e9,))
+    v, f = wl["meshes"][0]
+    kw = dict(wl["sweeps"][0][1], theta_start=theta, theta_end=theta, theta_samples=1, phi_start=phi, phi_end=phi,
+              phi_samples=1)
+    p = S.SweepParams.make(**kw)
+    mesh = ctx.mesh(v, f)
+    _step(f"airframe 3 GHz theta {theta} phi {phi}: mixed")
+    gm = ctx.trace_rays(mesh, p, 0, S.SBR_MIXED)
+    _step("fp64")
+    g6 = ctx.trace_rays(mesh, p, 0, S.SBR_FP64)
+    ctx.destroy_mesh(mesh)
+    assert np.array_equal(gm["count"], g6["count"])
+    assert np.array_equal(gm["ids"], g6["ids"])
+    assert np.array_equal(gm["L"], g6["L"])
+    print(f"    rays {len(gm['count'])}, re-traced {int(gm['retraced'].sum())}", flush=True)
+
+
+def test_plan_outliving_context_fails_cleanly():
+    c = S.Context(0)
+    v, f = M.icosphere(2)
+    mesh = c.mesh(v, f)
+    p = S.SweepParams.make(**dict(C1, phi_samples=4, phi_end=3, n_x=32, n_y=32))
+    pl = c.plan(S.sweep_jobs(p, mesh, S.SBR_FP32), S.SBR_FP32, S.SBR_BATCHED)
+    pl.run()
+    pl.wait()
+    c.close()
+    with pytest.raises(S.SbrError):
+        pl.run()
+    pl.close()

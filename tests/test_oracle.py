@@ -232,3 +232,13 @@ def test_reference_scan_against_mie_in_optical_region():
     sel = (x >= 30) & (x < 1000)
     err = np.abs(10 * np.log10(s[sel] / (np.pi * mie.normalized_rcs(x[sel]))))
     assert sel.sum() >= 40 and err.mean() < 0.06 and err.max() < 0.25
+
+
+@pytest.mark.parametrize("prec,size", [("f32", 4), ("f64", 8)])
+def test_reference_gpu_build_loads(prec, size):
+    """The reference's HIP kernels built from its sources (oracle/_ref) load
+    and report the precision they were built for (no GPU call)."""
+    L = O.ref_gpu_lib(prec)
+    if L is None:
+        pytest.skip("reference GPU library not built (needs the reference tree at build time)")
+    assert L.refgpu_real_size() == size
