@@ -183,6 +183,11 @@ int sbr_plan_run(sbr_plan* plan);  /* asynchronous on the context stream */
 int sbr_plan_wait(sbr_plan* plan);
 /* Device time of the last run's kernels in ms (hipEvents on the plan's stream). */
 int sbr_plan_last_ms(sbr_plan* plan, double* ms);
+/* The same interval split at the end of the trace launch(es): trace_ms =
+ * the ray-path kernel(s), rest_ms = the mixed mode's FP64 re-trace and
+ * finalize kernels. Under hipGraph replay the split is not recorded:
+ * trace_ms is the whole replay and rest_ms is 0. */
+int sbr_plan_last_split_ms(sbr_plan* plan, double* trace_ms, double* rest_ms);
 /* Per-job complex sum (2 doubles per job), sigma = 4*pi*|sum|^2 and dBsm
  * (src/app/sweep.cu:228-231); any output may be NULL. */
 int sbr_plan_results(sbr_plan* plan, double* reim, double* sigma, double* dbsm);

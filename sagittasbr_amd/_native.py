@@ -136,9 +136,10 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise SbrError(f"libsbr.so not built at {LIB_PATH}; run `make -C sagittasbr_amd` first")
-    L = C.CDLL(LIB_PATH)
+    path = os.environ.get("SAGITTASBR_LIB", LIB_PATH)  # another build of libsbr.so
+    if not os.path.exists(path):
+        raise SbrError(f"libsbr.so not built at {path}; run `make -C sagittasbr_amd` first")
+    L = C.CDLL(path)
     i32p = C.POINTER(C.c_int32)
     f64p = C.POINTER(C.c_double)
     sig = {
@@ -308,6 +309,12 @@ class Plan:
         ms = C.c_double()
         _check(lib().sbr_plan_last_ms(self.h, C.byref(ms)), self.ctx.h)
         return ms.value
+
+    def last_split_ms(self):
+        """(trace kernel ms, re-trace + finalize ms) of the last run."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib().sbr_plan_last_split_ms(self.h, C.byref(a), C.byref(b)), self.ctx.h)
+        return a.value, b.value
 
     def results(self):
         reim = np.zeros(2 * self.n, np.float64)

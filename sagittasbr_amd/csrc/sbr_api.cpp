@@ -133,7 +133,8 @@ struct sbr_plan {
     DevBuf dJobs64, dQueue, dQueueCount, dJobFix, dRecFlag;
     unsigned queueCap = 0;
     std::vector<double> out;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evMid = nullptr;  // evMid: after the trace launch(es)
+    bool splitValid = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     bool ran = false;
@@ -465,7 +466,8 @@ void release_plan_device(sbr_plan* pl) {
     drop_graph(pl);
     if (pl->ev0) (void)hipEventDestroy(pl->ev0);
     if (pl->ev1) (void)hipEventDestroy(pl->ev1);
-    pl->ev0 = pl->ev1 = nullptr;
+    if (pl->evMid) (void)hipEventDestroy(pl->evMid);
+    pl->ev0 = pl->ev1 = pl->evMid = nullptr;
     for (DevBuf* b : {&pl->dJobs, &pl->dPartials, &pl->dJobOut, &pl->dJobDone, &pl->dCounter, &pl->dCounters,
                       &pl->dJobs64, &pl->dQueue, &pl->dQueueCount, &pl->dJobFix, &pl->dRecFlag})
         b->release();
@@ -536,6 +538,8 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
             }
         }
     }
+    if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->evMid, s));
+    pl->splitValid = timed;
     if constexpr (std::is_same_v<R, float>) {
         if (mixed) {
             // FP64 re-traces of the flagged rays, then fold their fixed-point sums in
@@ -568,6 +572,7 @@ template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int
             HIP_TRY(&ctx->err, hipGraphInstantiate(&pl->graphExec, g, nullptr, nullptr, 0));
         }
         // events bracket the whole replay (fill kernels included)
+        pl->splitValid = false;
         HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
         HIP_TRY(&ctx->err, hipGraphLaunch(pl->graphExec, s));
         HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
@@ -621,9 +626,11 @@ int create_plan(sbr_ctx* ctx, const sbr_job* jobs, int n, int precision, int met
     pl->jobs.assign(jobs, jobs + n);
     HIP_TRY(&ctx->err, hipEventCreate(&pl->ev0));
     HIP_TRY(&ctx->err, hipEventCreate(&pl->ev1));
+    HIP_TRY(&ctx->err, hipEventCreate(&pl->evMid));
     if (dispatch_prepare(pl.get(), record)) {
         (void)hipEventDestroy(pl->ev0);
         (void)hipEventDestroy(pl->ev1);
+        (void)hipEventDestroy(pl->evMid);
         return 1;
     }
     ctx->plans.insert(pl.get());
@@ -986,6 +993,23 @@ int sbr_plan_last_ms(sbr_plan* pl, double* ms) {
     HIP_TRY(&pl->ctx->err, hipEventSynchronize(pl->ev1));
     HIP_TRY(&pl->ctx->err, hipEventElapsedTime(&f, pl->ev0, pl->ev1));
     *ms = f;
+    return 0;
+}
+
+int sbr_plan_last_split_ms(sbr_plan* pl, double* trace_ms, double* rest_ms) {
+    if (!pl || !trace_ms || !rest_ms) return fail(nullptr, "null argument");
+    if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
+    if (!pl->ran) return fail(&pl->ctx->err, "plan has not run");
+    HIP_TRY(&pl->ctx->err, hipEventSynchronize(pl->ev1));
+    float a = 0, b = 0;
+    if (pl->splitValid) {
+        HIP_TRY(&pl->ctx->err, hipEventElapsedTime(&a, pl->ev0, pl->evMid));
+        HIP_TRY(&pl->ctx->err, hipEventElapsedTime(&b, pl->evMid, pl->ev1));
+    } else {
+        HIP_TRY(&pl->ctx->err, hipEventElapsedTime(&a, pl->ev0, pl->ev1));
+    }
+    *trace_ms = a;
+    *rest_ms = b;
     return 0;
 }
 

@@ -718,8 +718,27 @@ constexpr int kPoolRays = 1024;  // rays per pool batch (4 passes of 16x16)
 // ends (one closest-hit query per loop iteration, so lanes of a wave never
 // wait for a longer path); each ray's term goes to its own LDS slot and the
 // slots are reduced per 16x16 pass exactly as without the pool.
+// Minimum waves per SIMD the register allocator must allow, per precision
+// mode (1 = no constraint); overridable at build time. Measured on the
+// trihedral+duct sweep and a 10 GHz airframe sweep (LDS stack 8): the
+// unconstrained mixed kernel takes ~150 VGPRs (3 waves) and runs 12-16%
+// slower than when held to 5 waves with a few spills; FP64 gains ~10% at
+// 4 waves, FP32 ~2-8% at 8. Deeper LDS stacks are bound by LDS instead.
+#ifndef SBR_WPE_MIXED
+#define SBR_WPE_MIXED 5
+#endif
+#ifndef SBR_WPE_FP64
+#define SBR_WPE_FP64 4
+#endif
+#ifndef SBR_WPE_FP32
+#define SBR_WPE_FP32 8
+#endif
+template <typename R, int MODE> constexpr int kWavesPerEu =
+    MODE == 1 ? SBR_WPE_MIXED : (std::is_same_v<R, double> ? SBR_WPE_FP64 : SBR_WPE_FP32);
+
 template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL>
-__global__ __launch_bounds__(kWgSize) void trace_kernel(TraceArgs<R> a) {
+__global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<R, MODE>))) void trace_kernel(
+    TraceArgs<R> a) {
     using ST = typename SlotOf<R, MODE>::T;
     using RS = typename std::conditional<MODE == 1, double, R>::type;  // ray state precision
     __shared__ int sRef[SD * kWgSize];

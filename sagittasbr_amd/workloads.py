@@ -14,6 +14,8 @@ grid_size / n_x is the stated fraction of a wavelength.
       8 bounces, theta = 60..120 step 2 x phi = 0..359 (11,160 per f)
   c4  64 seeded random bodies (5k-400k triangles each), own f ~ U[1, 10] GHz,
       lambda/10, 6 bounces, theta = 90, phi = 0..359 (23,040 angles)
+  c5  SH-perturbed sphere (icosphere 7, 327,680 triangles), 3 GHz, lambda/10,
+      4 bounces, 64 directions uniform on the sphere (the adjoint workload)
 """
 from __future__ import annotations
 
@@ -78,4 +80,22 @@ def c4(n_bodies=64):
                      f"lambda/10, max 6 bounces, theta 90, phi 0..359 ({360 * n_bodies:,} angles)")
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4}
+def c5_directions(n=64, seed=3):
+    """n directions uniform on the sphere (theta, phi in degrees)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    u, w = rng.random(n), rng.random(n)
+    return np.degrees(np.arccos(1 - 2 * u)), 360.0 * w
+
+
+def c5():
+    v, f, _ = M.sh_perturbed_sphere(7, seed=3)
+    th, ph = c5_directions()
+    return dict(name="c5", meshes=[(v, f)], directions=(th, ph),
+                job=dict(freq=3e9, grid_size=2.5, n_x=250, n_y=250, max_bounces=4),
+                desc=f"SH-perturbed sphere (icosphere 7, {len(f):,} triangles, 2 cm RMS), 3 GHz, 250x250 rays over "
+                     "2.5 m (lambda/10), max 4 bounces, 64 directions uniform on the sphere")
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}
