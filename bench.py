@@ -5,8 +5,8 @@ Workload (one step): trihedral corner reflector + open duct (34,816
 triangles), f = 10 GHz, lambda/10 ray spacing (1.6 m launch grid, 535 x 535
 rays), max 10 bounces, theta 0..90 x phi 0..90 in 1 degree steps (8,281
 angles), all angles in one persistent launch, mixed precision (FP32
-traversal with error bounds, FP64 ray state and phase, FP64 re-trace of the
-flagged rays). Mesh, BVH and the job list are resident in HBM before the
+traversal with error bounds, FP64 ray state and phase, queries that FP32
+cannot decide answered on the FP64 tree). Mesh, BVH and the job list are resident in HBM before the
 timed region; the launch shape is autotuned before it.
 
 Prints one JSON line: angles/s, ray-bounces/s, the plain per-angle FP32 and
@@ -175,7 +175,7 @@ def timed(plan, steps, warmup):
     for _ in range(steps):
         plan.run()
         plan.wait()
-        kernel_ms.append(plan.last_split_ms())
+        kernel_ms.append(plan.last_ms())
     torch.cuda.synchronize()
     return 1000.0 * (time.perf_counter() - t0) / steps, kernel_ms
 
@@ -255,7 +255,8 @@ def other_workload(args):
     log(f"{n} jobs, {rays:.3e} launched rays; autotuning")
     tuning = plan.autotune(sample_jobs=min(n, 96 if args.workload == "c3" else 512), repeats=1)
     if args.workload == "c4":
-        plan.tune(tuning["wg_per_cu"], tuning["lds_stack"], tuning["tile_rays"], 1, tuning["ray_pool"])
+        plan.tune(tuning["wg_per_cu"], tuning["lds_stack"], tuning["tile_rays"], 1, tuning["ray_pool"],
+                  tuning["traversal"])
         tuning["use_graph"] = 1
     log(f"tuning {tuning}")
     steps = args.steps if args.steps else 1
@@ -335,7 +336,8 @@ def main():
     n_angles = params.n_angles
     if args.tuning:
         t = json.loads(args.tuning)
-        plan.tune(t["wg_per_cu"], t["lds_stack"], t["tile_rays"], t["use_graph"], t.get("ray_pool", 0))
+        plan.tune(t["wg_per_cu"], t["lds_stack"], t["tile_rays"], t["use_graph"], t.get("ray_pool", 0),
+                  t.get("traversal", 0))
         tuning = t
     elif not args.no_autotune:
         tuning = plan.autotune(sample_jobs=0, repeats=1)
@@ -364,10 +366,8 @@ def main():
     derr = np.abs(dbsm - db64)
     strong = db64 > db64.max() - 30.0
     alg_bytes = cnt["node_visits"] * NODE_BYTES + cnt["tri_tests"] * TRI_BYTES + OUT_BYTES * n_angles
-    # the dominant kernel is the trace kernel; the mixed mode's FP64
-    # re-trace and finalize kernels follow it and are reported beside it
-    k_ms = float(np.mean([k[0] for k in kernel_ms]))
-    rest_ms = float(np.mean([k[1] for k in kernel_ms]))
+    # the trace kernel is the only kernel of a run besides the output fills
+    k_ms = float(np.mean(kernel_ms))
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
 
     # FP32 batched and the plain per-angle FP32 method (one launch per angle,
@@ -416,8 +416,7 @@ def main():
         "retraced_rays": own["retraced"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "trace_kernel", "kernel_ms": round(k_ms, 3), "algorithmic_bytes": int(alg_bytes),
-                     "retrace_and_finalize_ms": round(rest_ms, 3)},
+                     "kernel": "trace_kernel", "kernel_ms": round(k_ms, 3), "algorithmic_bytes": int(alg_bytes)},
     }
     if not args.no_pmc:
         log("rocprofv3 counter passes")

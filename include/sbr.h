@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define SBR_ABI_VERSION 1
+#define SBR_ABI_VERSION 2
 
 typedef struct sbr_ctx sbr_ctx;
 typedef struct sbr_plan sbr_plan;
@@ -88,6 +88,9 @@ typedef struct {
     int use_graph;   /* 1: capture the run's launches in a hipGraph once and replay it */
     int ray_pool;    /* 1: lanes refill from the tile's ray pool after each ray;
                         0 or 2: one ray per lane per 16 x 16 pass (default) */
+    int traversal;   /* loop shape of the BVH traversal: 1 = if-if (a wavefront runs node
+                        steps and leaf tests side by side), 2 = while-while (leaves are
+                        tested once every lane has reached one; default) */
 } sbr_tuning;
 
 /* Mesh / BVH statistics. */
@@ -107,7 +110,8 @@ typedef struct {
     uint64_t tri_tests;   /* Moller-Trumbore tests */
     uint64_t ray_bounces; /* sum over rays of hitCount */
     uint64_t queries;     /* closest-hit queries */
-    uint64_t retraced;    /* mixed mode: rays re-traced in FP64 after an ambiguous FP32 decision */
+    uint64_t retraced;    /* mixed mode: rays with a query answered on the FP64 tree (an FP32
+                             decision fell inside its error bound) */
 } sbr_counters;
 
 const char* sbr_last_error(const sbr_ctx* ctx);
@@ -181,13 +185,10 @@ int sbr_plan_create(sbr_ctx* ctx, const sbr_job* jobs, int n_jobs, int precision
 int sbr_plan_set_tuning(sbr_plan* plan, const sbr_tuning* t);
 int sbr_plan_run(sbr_plan* plan);  /* asynchronous on the context stream */
 int sbr_plan_wait(sbr_plan* plan);
-/* Device time of the last run's kernels in ms (hipEvents on the plan's stream). */
+/* Device time of the last run's ray-path kernel launch(es) in ms (hipEvents
+ * on the plan's stream; under hipGraph replay the whole replay, output
+ * clearing included). */
 int sbr_plan_last_ms(sbr_plan* plan, double* ms);
-/* The same interval split at the end of the trace launch(es): trace_ms =
- * the ray-path kernel(s), rest_ms = the mixed mode's FP64 re-trace and
- * finalize kernels. Under hipGraph replay the split is not recorded:
- * trace_ms is the whole replay and rest_ms is 0. */
-int sbr_plan_last_split_ms(sbr_plan* plan, double* trace_ms, double* rest_ms);
 /* Per-job complex sum (2 doubles per job), sigma = 4*pi*|sum|^2 and dBsm
  * (src/app/sweep.cu:228-231); any output may be NULL. */
 int sbr_plan_results(sbr_plan* plan, double* reim, double* sigma, double* dbsm);
@@ -217,7 +218,7 @@ int sbr_sweep(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precisio
  * (idx = j*n_x + i). Outputs (any may be NULL): hit_ids[n_rays*max_bounces]
  * original face index per bounce (-1 after the last hit); bounces[n_rays];
  * path_len[n_rays]; reim_ray[2*n_rays] PO term per ray; retraced[n_rays]
- * (SBR_MIXED) 1 where the ray was re-traced in FP64. */
+ * (SBR_MIXED) 1 where a query of the ray was answered on the FP64 tree. */
 int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int angle, int precision, int32_t* hit_ids,
                    int32_t* bounces, double* path_len, double* reim_ray, int8_t* retraced);
 

@@ -25,12 +25,9 @@
 
 namespace sbr {
 template <typename R>
-hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool, int grid,
-                        hipStream_t s);
+hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool, bool ww,
+                        int grid, hipStream_t s);
 hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s);
-hipError_t launch_retrace(const TraceArgs<float>& a, bool rec, bool cnt, int grid, hipStream_t s);
-hipError_t launch_finalize(const Job<double>* jobs64, int n, const unsigned long long* fix, double* out,
-                           hipStream_t s);
 size_t partial_bytes(bool fp64, int mode);
 hipError_t launch_adjoint(const AdjArgs& a, hipStream_t s);
 template <typename R> int trace_occupancy(int mode, int sd, bool pool);
@@ -130,11 +127,9 @@ struct sbr_plan {
     int tileSide = 32;
     DevBuf dJobs, dPartials, dJobOut, dJobDone, dCounter, dCounters;
     // mixed mode
-    DevBuf dJobs64, dQueue, dQueueCount, dJobFix, dRecFlag;
-    unsigned queueCap = 0;
+    DevBuf dJobs64, dRecFlag;
     std::vector<double> out;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evMid = nullptr;  // evMid: after the trace launch(es)
-    bool splitValid = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     bool ran = false;
@@ -360,8 +355,6 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
             D.ny = j.n_y;
             D.maxB = j.max_bounces;
             D.mesh = j.mesh;
-            const double rmax = std::max(1.0, std::fabs(j.reflection_const));
-            D.fixScale = std::ldexp(1.0, 36) / (std::fabs(D.pre) * std::pow(rmax, j.max_bounces) + 1e-300);
             // FP32 job: FP64 frame moved towards the body, rounded once
             const double off = 10.0 * j.grid_size;
             // keep >= 1 cm (> tMin) between the moved plane and any vertex
@@ -382,7 +375,6 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
             J.kd = D.kd;
             J.twoPid = D.twoPid;
             J.L0 = shift;
-            J.fixScale = D.fixScale;
             auto nrm = [](const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
             J.ex0 = float(kErrSafety * 5.9604645e-8 * 4.0 * (nrm(llc2) + nrm(D.uv) + nrm(D.vv)));
             jobs64[i] = D;
@@ -434,18 +426,15 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         HIP_TRY(&ctx->err, pl->dJobs64.ensure(sizeof(Job<double>) * std::max(pl->nJobs, 1)));
         HIP_TRY(&ctx->err,
                 hipMemcpy(pl->dJobs64.p, jobs64.data(), sizeof(Job<double>) * pl->nJobs, hipMemcpyHostToDevice));
-        // re-trace queue: room for every traced ray up to 32M entries; beyond
-        // that the trace kernel re-traces inline
-        int64_t cap = std::min<int64_t>(std::max<int64_t>(rays, 1), int64_t(1) << 25);
-        pl->queueCap = unsigned(cap);
-        HIP_TRY(&ctx->err, pl->dQueue.ensure(sizeof(QEntry) * size_t(cap)));
-        HIP_TRY(&ctx->err, pl->dQueueCount.ensure(sizeof(unsigned) * 2));
-        HIP_TRY(&ctx->err, pl->dJobFix.ensure(sizeof(unsigned long long) * 2 * std::max(pl->nJobs, 1)));
     }
     return 0;
 }
 
 int default_sd(int /*depth*/) { return 8; }
+
+// While-while traversal: measured faster where the BVH stays in cache (the
+// trihedral+duct sweep); large meshes can prefer if-if (the autotuner tries both).
+int default_traversal() { return 2; }
 
 // One angle launched on its own fills the GPU only with small tiles (a
 // 535 x 535 grid gives ~1,100 tiles of 16 x 16 but ~280 of 32 x 32); the
@@ -466,10 +455,9 @@ void release_plan_device(sbr_plan* pl) {
     drop_graph(pl);
     if (pl->ev0) (void)hipEventDestroy(pl->ev0);
     if (pl->ev1) (void)hipEventDestroy(pl->ev1);
-    if (pl->evMid) (void)hipEventDestroy(pl->evMid);
-    pl->ev0 = pl->ev1 = pl->evMid = nullptr;
+    pl->ev0 = pl->ev1 = nullptr;
     for (DevBuf* b : {&pl->dJobs, &pl->dPartials, &pl->dJobOut, &pl->dJobDone, &pl->dCounter, &pl->dCounters,
-                      &pl->dJobs64, &pl->dQueue, &pl->dQueueCount, &pl->dJobFix, &pl->dRecFlag})
+                      &pl->dJobs64, &pl->dRecFlag})
         b->release();
     pl->ctx = nullptr;
 }
@@ -490,6 +478,7 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     a.meshes = static_cast<const Mesh<R>*>(sizeof(R) == 4 ? ctx->meshTable32.p : ctx->meshTable64.p);
     a.nJobs = pl->nJobs;
     a.tileSide = pl->tileSide;
+    const bool ww = (pl->tune.traversal ? pl->tune.traversal : default_traversal()) == 2;
     a.totalTiles = pl->totalTiles;
     a.partials = pl->dPartials.p;
     a.jobOut = static_cast<double*>(pl->dJobOut.p);
@@ -504,11 +493,7 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     if (mixed) {
         a.jobs64 = static_cast<const Job<double>*>(pl->dJobs64.p);
         a.meshes64 = static_cast<const Mesh<double>*>(ctx->meshTable64.p);
-        a.queue = static_cast<QEntry*>(pl->dQueue.p);
-        a.queueCount = static_cast<unsigned*>(pl->dQueueCount.p);
-        a.queueCap = pl->queueCap;
         a.errK = kErrSafety;
-        a.jobFix = static_cast<unsigned long long*>(pl->dJobFix.p);
         a.recFlag = static_cast<int8_t*>(pl->dRecFlag.p);
     }
     hipStream_t s = ctx->stream;
@@ -516,10 +501,6 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobDone.p), size_t(pl->nJobs), s));
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounter.p), 2, s));
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounters.p), 32, s));
-    if (mixed) {
-        HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dQueueCount.p), 2, s));
-        HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobFix.p), size_t(4) * pl->nJobs, s));
-    }
     // the timed region covers the trace launches only
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
     if (pl->totalTiles > 0) {
@@ -529,22 +510,14 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
             int64_t grid = int64_t(ctx->cuCount) * per;
             grid = std::min<int64_t>(grid, pl->totalTiles);
             a.tileOffset = 0;
-            HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, false, count, true, pool, int(grid), s));
+            HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, false, count, true, pool, ww, int(grid), s));
         } else {
             for (int i = 0; i < pl->nJobs; ++i) {
                 if (pl->nTiles[i] == 0) continue;
                 a.tileOffset = pl->tileBase[i];
-                HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, record, count, false, pool, pl->nTiles[i], s));
+                HIP_TRY(&ctx->err,
+                        launch_trace<R>(a, mode, sd, record, count, false, pool, ww, pl->nTiles[i], s));
             }
-        }
-    }
-    if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->evMid, s));
-    pl->splitValid = timed;
-    if constexpr (std::is_same_v<R, float>) {
-        if (mixed) {
-            // FP64 re-traces of the flagged rays, then fold their fixed-point sums in
-            HIP_TRY(&ctx->err, launch_retrace(a, record, count, ctx->cuCount * 8, s));
-            HIP_TRY(&ctx->err, launch_finalize(a.jobs64, pl->nJobs, a.jobFix, a.jobOut, s));
         }
     }
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
@@ -572,7 +545,6 @@ template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int
             HIP_TRY(&ctx->err, hipGraphInstantiate(&pl->graphExec, g, nullptr, nullptr, 0));
         }
         // events bracket the whole replay (fill kernels included)
-        pl->splitValid = false;
         HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
         HIP_TRY(&ctx->err, hipGraphLaunch(pl->graphExec, s));
         HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
@@ -626,11 +598,9 @@ int create_plan(sbr_ctx* ctx, const sbr_job* jobs, int n, int precision, int met
     pl->jobs.assign(jobs, jobs + n);
     HIP_TRY(&ctx->err, hipEventCreate(&pl->ev0));
     HIP_TRY(&ctx->err, hipEventCreate(&pl->ev1));
-    HIP_TRY(&ctx->err, hipEventCreate(&pl->evMid));
     if (dispatch_prepare(pl.get(), record)) {
         (void)hipEventDestroy(pl->ev0);
         (void)hipEventDestroy(pl->ev1);
-        (void)hipEventDestroy(pl->evMid);
         return 1;
     }
     ctx->plans.insert(pl.get());
@@ -692,6 +662,8 @@ int sbr_mesh_create(sbr_ctx* ctx, const float* xyz, int n_verts, const uint32_t*
     if (!ctx) return fail(nullptr, "null context");
     if (!xyz || !idx || !mesh_id || n_verts <= 0 || n_tris <= 0)
         return fail(&ctx->err, "mesh contains no valid triangles");
+    // leaf references pack the first triangle in 27 bits
+    if (n_tris >= (1 << 27)) return fail(&ctx->err, "meshes are limited to 2^27 - 1 triangles");
     for (int64_t i = 0; i < int64_t(n_tris) * 3; ++i)
         if (idx[i] >= uint32_t(n_verts)) return fail(&ctx->err, "mesh contains out-of-range indices");
     auto m = std::make_unique<MeshEntry>();
@@ -959,6 +931,7 @@ int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
     if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     if (t->lds_stack < 0 || t->wg_per_cu < 0) return fail(&pl->ctx->err, "negative tuning value");
     if (t->ray_pool < 0 || t->ray_pool > 2) return fail(&pl->ctx->err, "ray_pool must be 0, 1 or 2");
+    if (t->traversal < 0 || t->traversal > 2) return fail(&pl->ctx->err, "traversal must be 0, 1 or 2");
     if (t->tile_rays != 0 && t->tile_rays != 16 && t->tile_rays != 32 && t->tile_rays != 64)
         return fail(&pl->ctx->err, "tile_rays must be 16, 32 or 64");
     drop_graph(pl);
@@ -996,23 +969,6 @@ int sbr_plan_last_ms(sbr_plan* pl, double* ms) {
     return 0;
 }
 
-int sbr_plan_last_split_ms(sbr_plan* pl, double* trace_ms, double* rest_ms) {
-    if (!pl || !trace_ms || !rest_ms) return fail(nullptr, "null argument");
-    if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
-    if (!pl->ran) return fail(&pl->ctx->err, "plan has not run");
-    HIP_TRY(&pl->ctx->err, hipEventSynchronize(pl->ev1));
-    float a = 0, b = 0;
-    if (pl->splitValid) {
-        HIP_TRY(&pl->ctx->err, hipEventElapsedTime(&a, pl->ev0, pl->evMid));
-        HIP_TRY(&pl->ctx->err, hipEventElapsedTime(&b, pl->evMid, pl->ev1));
-    } else {
-        HIP_TRY(&pl->ctx->err, hipEventElapsedTime(&a, pl->ev0, pl->ev1));
-    }
-    *trace_ms = a;
-    *rest_ms = b;
-    return 0;
-}
-
 int sbr_plan_results(sbr_plan* pl, double* reim, double* sigma, double* dbsm) {
     if (!pl) return fail(nullptr, "null plan");
     if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
@@ -1046,13 +1002,13 @@ int sbr_plan_count(sbr_plan* pl, sbr_counters* out) {
     unsigned long long c[16] = {0};
     HIP_TRY(&ctx->err, hipMemcpyAsync(c, pl->dCounters.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
-    // [0..5] trace kernel: rays, visits, tests, bounces, queries, re-traced;
-    // [6..9] FP64 re-traces: visits, tests, bounces, queries
+    // rays, visits, tests, bounces, queries (each query counted once, FP64
+    // answers of undecided queries included), rays with an FP64-answered query
     out->rays = c[0];
-    out->node_visits = c[1] + c[6];
-    out->tri_tests = c[2] + c[7];
-    out->ray_bounces = c[3] + c[8];
-    out->queries = c[4] + c[9];
+    out->node_visits = c[1];
+    out->tri_tests = c[2];
+    out->ray_bounces = c[3];
+    out->queries = c[4];
     out->retraced = c[5];
     return 0;
 }
@@ -1073,42 +1029,54 @@ int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* be
         if (create_plan(ctx, sub.data(), int(sub.size()), pl->precision, pl->method, false, &owned)) return 1;
         probe = owned;
     }
+    // stage 1: launch shape at the default traversal loop shape
     std::vector<sbr_tuning> cands;
     for (int pool : {1, 2})
         for (int sd : {8, 16, 32}) {
             if (pl->method == SBR_BATCHED) {
                 for (int ts : {16, 32, 64})
-                    for (int wg : {0, 2}) cands.push_back(sbr_tuning{wg, sd, ts, 0, pool});
+                    for (int wg : {0, 2}) cands.push_back(sbr_tuning{wg, sd, ts, 0, pool, 0});
             } else {
                 for (int ts : {16, 32})
-                    for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool});
+                    for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool, 0});
             }
         }
     double bestT = 1e300;
     sbr_tuning bestC = cands[0];
-    int rc = 0;
-    for (const sbr_tuning& c : cands) {
-        if ((rc = sbr_plan_set_tuning(probe, &c))) break;
+    int rc = 0, tried = 0;
+    auto trial = [&](const sbr_tuning& c) -> int {
+        if (int e = sbr_plan_set_tuning(probe, &c)) return e;
         // one untimed run (graph capture, caches), then the timed ones
-        if ((rc = sbr_plan_run(probe)) || (rc = sbr_plan_wait(probe))) break;
+        if (int e = sbr_plan_run(probe)) return e;
+        if (int e = sbr_plan_wait(probe)) return e;
         double t = 1e300;
         for (int r = 0; r < repeats; ++r) {
             auto t0 = std::chrono::steady_clock::now();
-            if ((rc = sbr_plan_run(probe)) || (rc = sbr_plan_wait(probe))) break;
+            if (int e = sbr_plan_run(probe)) return e;
+            if (int e = sbr_plan_wait(probe)) return e;
             t = std::min(t, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
         }
-        if (rc) break;
+        ++tried;
         if (t < bestT) {
             bestT = t;
             bestC = c;
         }
+        return 0;
+    };
+    for (const sbr_tuning& c : cands)
+        if ((rc = trial(c))) break;
+    // stage 2: the other traversal loop shape at the best launch shape
+    if (!rc) {
+        sbr_tuning c = bestC;
+        c.traversal = default_traversal() == 2 ? 1 : 2;
+        rc = trial(c);
     }
     if (owned) sbr_plan_destroy(owned);
     if (rc) return rc;
     if (sbr_plan_set_tuning(pl, &bestC)) return 1;
     if (best) *best = bestC;
     if (best_ms) *best_ms = bestT;
-    if (n_tried) *n_tried = int(cands.size());
+    if (n_tried) *n_tried = tried;
     return 0;
 }
 

@@ -19,7 +19,6 @@ template <typename R> struct alignas(16) Job {
     R k, pre, twoPi, refl;           // k, (k*dA/4pi)*2, 2pi, reflection const
     double kd, twoPid;
     double L0;        // mixed mode: distance the launch plane was moved towards the body
-    double fixScale;  // mixed mode: fixed-point scale of FP64 re-trace contributions
     float ex0;        // mixed mode: initial ray position error bound [m]
     int nx, ny, maxB, mesh;
     int i0, j0, tilesX, tilesY;      // culled ray rectangle in tiles
@@ -44,18 +43,12 @@ template <typename R> struct alignas(16) Mesh {
     int nTris;
 };
 
-// FP64 re-trace request of the mixed mode
-struct QEntry {
-    int job;
-    int ray;
-};
-
 // Kernel arguments (passed by value).
 template <typename R> struct TraceArgs {
     const Job<R>* jobs;
     const Mesh<R>* meshes;
     int nJobs;
-    int tileSide;            // rays per tile side (8, 16 or 32)
+    int tileSide;            // rays per tile side (16, 32 or 64)
     int64_t totalTiles;
     int64_t tileOffset;      // first tile of a non-persistent launch
     void* partials;          // one accumulator per 16x16 ray pass
@@ -67,17 +60,15 @@ template <typename R> struct TraceArgs {
     int32_t* recCount;
     double* recL;
     double* recReIm;
-    // work counters: rays, visits, tests, bounces, queries, re-traced rays
+    // work counters: rays, visits, tests, bounces, queries, rays with a
+    // query answered on the FP64 tree
     unsigned long long* counters;
-    // mixed mode: reference FP64 jobs and meshes for re-traces, the re-trace
-    // queue, fixed-point per-job accumulators, per-ray re-trace flags
+    // mixed mode: reference FP64 jobs and meshes (FP64 ray state and the
+    // queries that FP32 cannot decide), error-bound safety factor, per-ray
+    // flags of rays with such a query (records only)
     const Job<double>* jobs64;
     const Mesh<double>* meshes64;
-    QEntry* queue;
-    unsigned* queueCount;
-    unsigned queueCap;
     float errK;
-    unsigned long long* jobFix;
     int8_t* recFlag;
 };
 

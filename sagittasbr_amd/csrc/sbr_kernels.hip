@@ -30,15 +30,15 @@
 // Mixed mode (MODE 1): FP32 traversal whose every relevant decision carries
 // a forward error bound against the FP64 reference ray (filtered
 // predicates). Boxes are tested grown by the ray's position error bound, so
-// pruning is conservative; Moller-Trumbore decisions whose barycentric,
-// determinant or distance margin falls inside the bound flag the ray. A
-// flagged ray contributes nothing in FP32 and is re-traced by the FP64
-// reference path (queue kernel), whose contribution is accumulated in
-// 64-bit fixed point (order independent, hence deterministic). Unflagged
-// rays therefore have the FP64 outcome; path length and phase are FP64 and
-// the complex sum is FP32 two-sum compensated.
+// pruning is conservative; a query with a Moller-Trumbore decision whose
+// barycentric, determinant or distance margin falls inside the bound is
+// answered instead by the reference FP64 query on the FP64 tree, from the
+// same FP64 ray state. Every query therefore has the FP64 outcome; the ray
+// state, path length and phase are FP64 and the complex sum is FP32
+// two-sum compensated.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <type_traits>
 
 #include "sbr_device.hpp"
@@ -98,6 +98,22 @@ __device__ __forceinline__ bool slab(R lx, R ly, R lz, R hx, R hy, R hz, V3<R> o
     R tn = mx(mx(mn(t0x, t1x), mn(t0y, t1y)), mn(t0z, t1z));
     R tf = mn(mn(mx(t0x, t1x), mx(t0y, t1y)), mx(t0z, t1z));
     entry = mx(tn, R(0.001));
+    return entry <= tf;
+}
+
+// Mixed-mode slab test of the box grown by dx on every side, one FMA per
+// plane: (lo - dx - o) * inv = lo * inv - oLo * inv with oLo = o + dx, and
+// (hi + dx - o) * inv = hi * inv - oHi * inv with oHi = o - dx (oLo, oHi
+// rounded outwards and pLo = oLo * inv, pHi = oHi * inv formed once per
+// query by the caller; inv is finite). Its rounding, eps (|o| + |t|) in
+// position units, is covered by the position error bound dx.
+__device__ __forceinline__ bool slab_grown(float lx, float ly, float lz, float hx, float hy, float hz, V3<float> pLo,
+                                           V3<float> pHi, V3<float> inv, float& entry) {
+    float t0x = fmaf(lx, inv.x, -pLo.x), t0y = fmaf(ly, inv.y, -pLo.y), t0z = fmaf(lz, inv.z, -pLo.z);
+    float t1x = fmaf(hx, inv.x, -pHi.x), t1y = fmaf(hy, inv.y, -pHi.y), t1z = fmaf(hz, inv.z, -pHi.z);
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    entry = fmaxf(tn, 0.001f);
     return entry <= tf;
 }
 
@@ -216,62 +232,29 @@ struct Work {
     unsigned visits, tests;
 };
 
-// Closest hit (rtKernels.cu:63-100). Returns the BVH position of the hit
-// triangle or -1. The deferred-left stack lives in LDS: entry k of thread
-// tid at sRef[k*kWgSize + tid] / sE[...].
-// Entries beyond SD spill to the thread's private overflow arrays (scratch);
-// the host guarantees tree depth <= kMaxDepth so sp < kMaxStack.
-template <typename R, int SD, bool CNT>
-__device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V3<R> inv, R& tHit, int* sRef, R* sE,
-                                           typename Ov<R>::T* ov, int tid, Work& w) {
-    const R* nodes = static_cast<const R*>(m.nodes);
-    const R* tris = static_cast<const R*>(m.tris);
-    R closest = R(1e20);
-    int hit = -1;
-    if (CNT) w.visits += 1;
-    R e;
-    if (!slab(m.rootLo[0], m.rootLo[1], m.rootLo[2], m.rootHi[0], m.rootHi[1], m.rootHi[2], o, inv, e)) return -1;
-    int cur = m.rootRef;
-    int sp = 0;
-    for (;;) {
-        if (cur >= 0) {
-            NodeRec<R> n = load_node(nodes, cur);
-            if (CNT) w.visits += 2;
-            R eL, eR;
-            bool hL = slab(n.b[0], n.b[1], n.b[2], n.b[3], n.b[4], n.b[5], o, inv, eL) && eL <= closest;
-            bool hR = slab(n.b[6], n.b[7], n.b[8], n.b[9], n.b[10], n.b[11], o, inv, eR) && eR <= closest;
-            if (hR) {
-                if (hL) {
-                    if (SD > 0 && sp < SD) {
-                        sRef[sp * kWgSize + tid] = n.l;
-                        sE[sp * kWgSize + tid] = eL;
-                    } else {
-                        ov[sp - SD] = Ov<R>::pack(eL, n.l);
-                    }
-                    ++sp;
-                }
-                cur = n.r;
-                continue;
-            }
-            if (hL) {
-                cur = n.l;
-                continue;
-            }
+// Reference value meaning "stack exhausted": below every leaf code
+// ~((first << 4) | (n - 1)) because the host keeps first < 2^27.
+constexpr int kNoRef = INT_MIN;
+
+// Deferred-left stack of one lane: entries [0, SD) in LDS at
+// [k * kWgSize + tid], deeper ones in the private overflow array.
+template <typename R, int SD> struct LaneStack {
+    int* sRef;
+    R* sE;
+    typename Ov<R>::T* ov;
+    int tid;
+    int sp;
+    __device__ __forceinline__ void push(R e, int r) {
+        if (SD > 0 && sp < SD) {
+            sRef[sp * kWgSize + tid] = r;
+            sE[sp * kWgSize + tid] = e;
         } else {
-            int code = ~cur;
-            int first = code >> 4;
-            int cnt = (code & 15) + 1;
-            if (CNT) w.tests += cnt;
-            for (int k = 0; k < cnt; ++k) {
-                TriRec<R> tr = load_tri(tris, first + k);
-                R t;
-                if (mt_test(tr, o, d, closest, t)) {
-                    closest = t;
-                    hit = first + k;
-                }
-            }
+            ov[sp - SD] = Ov<R>::pack(e, r);
         }
-        bool found = false;
+        ++sp;
+    }
+    // the next deferred child whose slab entry is <= lim, or kNoRef
+    __device__ __forceinline__ int pop(R lim) {
         while (sp > 0) {
             --sp;
             R ee;
@@ -282,13 +265,88 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
             } else {
                 Ov<R>::unpack(ov[sp - SD], ee, rr);
             }
-            if (ee <= closest) {
-                cur = rr;
-                found = true;
-                break;
+            if (ee <= lim) return rr;
+        }
+        return kNoRef;
+    }
+};
+
+// Loop shapes of the traversal (WW template flag). If-if (false): every
+// iteration each lane takes its next step, a node step or a leaf's triangle
+// tests, so a wavefront runs both kinds side by side and their memory
+// latencies overlap. While-while (true): lanes walk internal nodes (pops
+// included) until each has reached a leaf or emptied its stack, then the
+// leaves are tested together, so the triangle code runs once for all lanes
+// instead of beside every node step. Each lane's own sequence of node
+// tests, triangle tests and pops -- and so every decision -- is that of the
+// reference loop in both shapes.
+// Closest hit (rtKernels.cu:63-100). Returns the BVH position of the hit
+// triangle or -1. The host guarantees tree depth <= kMaxDepth so the stack
+// stays below kMaxStack entries.
+template <typename R, int SD, bool CNT, bool WW>
+__device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V3<R> inv, R& tHit, int* sRef, R* sE,
+                                           typename Ov<R>::T* ov, int tid, Work& w) {
+    const R* nodes = static_cast<const R*>(m.nodes);
+    const R* tris = static_cast<const R*>(m.tris);
+    R closest = R(1e20);
+    int hit = -1;
+    if (CNT) w.visits += 1;
+    R e;
+    if (!slab(m.rootLo[0], m.rootLo[1], m.rootLo[2], m.rootHi[0], m.rootHi[1], m.rootHi[2], o, inv, e)) return -1;
+    LaneStack<R, SD> st{sRef, sE, ov, tid, 0};
+    int cur = m.rootRef;
+    // one node step: descend into a hit child (true) or not (false)
+    auto node_step = [&]() -> bool {
+        NodeRec<R> n = load_node(nodes, cur);
+        if (CNT) w.visits += 2;
+        R eL, eR;
+        bool hL = slab(n.b[0], n.b[1], n.b[2], n.b[3], n.b[4], n.b[5], o, inv, eL) && eL <= closest;
+        bool hR = slab(n.b[6], n.b[7], n.b[8], n.b[9], n.b[10], n.b[11], o, inv, eR) && eR <= closest;
+        if (hR) {
+            if (hL) st.push(eL, n.l);
+            cur = n.r;
+            return true;
+        }
+        if (hL) {
+            cur = n.l;
+            return true;
+        }
+        return false;
+    };
+    auto leaf_tests = [&]() {
+        int code = ~cur;
+        int first = code >> 4;
+        int cnt = (code & 15) + 1;
+        if (CNT) w.tests += cnt;
+        for (int k = 0; k < cnt; ++k) {
+            TriRec<R> tr = load_tri(tris, first + k);
+            R t;
+            if (mt_test(tr, o, d, closest, t)) {
+                closest = t;
+                hit = first + k;
             }
         }
-        if (!found) break;
+    };
+    if constexpr (WW) {
+        for (;;) {
+            while (cur >= 0)
+                if (!node_step()) cur = st.pop(closest);
+            if (cur == kNoRef) break;
+            leaf_tests();
+            cur = st.pop(closest);
+            if (cur == kNoRef) break;
+        }
+    } else {
+        // one pop for the lanes that left a leaf and those at a dead end
+        for (;;) {
+            if (cur >= 0) {
+                if (node_step()) continue;
+            } else {
+                leaf_tests();
+            }
+            cur = st.pop(closest);
+            if (cur == kNoRef) break;
+        }
     }
     tHit = closest;
     return hit;
@@ -297,7 +355,7 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
 // ------------------------------------------------- mixed-mode traversal
 constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
 
-template <int SD, bool CNT>
+template <int SD, bool CNT, bool WW>
 __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> d, V3<float> inv,
                                                  float& tHit, int* sRef, float* sE, Ov<float>::T* ov, int tid,
                                                  Work& w, float ex, float ed, float K, bool& amb) {
@@ -313,111 +371,107 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     // position and distance error bounds valid over the whole query
     float dx = ex + ed * tFar + K * kEps32 * (o1 + tFar);
     float tErr = dx + 4.0f * K * kEps32 * tFar;
-    const float tMinE = 0.001f - tErr;
     float closest = 1e20f, tBestErr = 0.0f;
     float tAmb = 3.0e38f, t2 = 3.0e38f;
     int hit = -1;
     float e;
     if (CNT) w.visits += 1;
-    if (!slab(m.rootLo[0] - dx, m.rootLo[1] - dx, m.rootLo[2] - dx, m.rootHi[0] + dx, m.rootHi[1] + dx,
-              m.rootHi[2] + dx, o, inv, e)) {
+    // o + dx and o - dx rounded outwards: the margin covers the rounding of
+    // the sums and of the margin itself
+    const float dxs = dx * (1.0f + 8.0f * kEps32) + 4.0f * kEps32 * o1;
+    const V3<float> pLo{(o.x + dxs) * inv.x, (o.y + dxs) * inv.y, (o.z + dxs) * inv.z};
+    const V3<float> pHi{(o.x - dxs) * inv.x, (o.y - dxs) * inv.y, (o.z - dxs) * inv.z};
+    if (!slab_grown(m.rootLo[0], m.rootLo[1], m.rootLo[2], m.rootHi[0], m.rootHi[1], m.rootHi[2], pLo, pHi, inv,
+                    e)) {
         tHit = closest;
         return -1;
     }
-    (void)tMinE;
+    LaneStack<float, SD> st{sRef, sE, ov, tid, 0};
     int cur = m.rootRef;
-    int sp = 0;
-    for (;;) {
-        // the true closest distance may lie up to tBestErr beyond the FP32
-        // one; a box is pruned only when it starts beyond that as well
+    // the true closest distance may lie up to tBestErr beyond the FP32 one;
+    // a box is pruned only when it starts beyond that as well
+    auto node_step = [&]() -> bool {
         const float lim = closest + tBestErr + tErr;
-        if (cur >= 0) {
-            NodeRec<float> n = load_node(nodes, cur);
-            if (CNT) w.visits += 2;
-            float eL, eR;
-            bool hL = slab(n.b[0] - dx, n.b[1] - dx, n.b[2] - dx, n.b[3] + dx, n.b[4] + dx, n.b[5] + dx, o, inv, eL) &&
-                      eL <= lim;
-            bool hR = slab(n.b[6] - dx, n.b[7] - dx, n.b[8] - dx, n.b[9] + dx, n.b[10] + dx, n.b[11] + dx, o, inv,
-                           eR) &&
-                      eR <= lim;
-            if (hR) {
-                if (hL) {
-                    if (sp < SD) {
-                        sRef[sp * kWgSize + tid] = n.l;
-                        sE[sp * kWgSize + tid] = eL;
-                    } else {
-                        ov[sp - SD] = Ov<float>::pack(eL, n.l);
-                    }
-                    ++sp;
-                }
-                cur = n.r;
-                continue;
-            }
-            if (hL) {
-                cur = n.l;
-                continue;
-            }
-        } else {
-            int code = ~cur;
-            int first = code >> 4;
-            int cnt = (code & 15) + 1;
-            if (CNT) w.tests += cnt;
-            for (int k = 0; k < cnt; ++k) {
-                TriRec<float> tr = load_tri(tris, first + k);
-                V3<float> p = cross3(d, tr.e2);
-                float det = dot3(tr.e1, p);
-                float adet = fabsf(det);
-                float invd = 1.0f / det;
-                V3<float> tv = sub(o, tr.v0);
-                float u = dot3(tv, p) * invd;
-                V3<float> q = cross3(tv, tr.e1);
-                float v = dot3(d, q) * invd;
-                float t = dot3(tr.e2, q) * invd;
-                // forward error bounds of this test against the FP64 ray
-                float at = fabsf(t);
-                float v01 = fabsf(tr.v0.x) + fabsf(tr.v0.y) + fabsf(tr.v0.z);
-                float xerr = ex + ed * at + K * kEps32 * (o1 + v01 + 1.0f);
-                float rdet = 1.0f / fmaxf(adet, 1e-30f);
-                float berr = xerr * tr.sumE * rdet + 4.0f * K * kEps32;
-                float terr = xerr * tr.prodE * rdet + K * kEps32 * (o1 + v01 + at);
-                float ddet = (ed + K * kEps32) * tr.prodE * 2.0f;
-                float mrg = fminf(u, fminf(v, 1.0f - u - v));
-                bool detOk = adet >= 1.0e-6f;
-                bool inside = detOk && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || u + v > 1.0f);
-                bool nearEdge = detOk && fabsf(mrg) <= berr;
-                bool ambig = (fabsf(adet - 1.0e-6f) <= ddet) || nearEdge ||
-                             ((inside || nearEdge) && fabsf(t - 0.001f) <= terr);
-                if (ambig && t >= 0.001f - terr) tAmb = fminf(tAmb, t - terr);
-                // t2: lowest possible true distance of any other hit (each
-                // competitor with its own error bound)
-                if (inside && t >= 0.001f && t <= closest) {
-                    if (closest < 1e19f) t2 = fminf(t2, closest - tBestErr);
-                    closest = t;
-                    tBestErr = terr;
-                    hit = first + k;
-                } else if (inside && t > closest) {
-                    t2 = fminf(t2, t - terr);
-                }
+        NodeRec<float> n = load_node(nodes, cur);
+        if (CNT) w.visits += 2;
+        float eL, eR;
+        bool hL = slab_grown(n.b[0], n.b[1], n.b[2], n.b[3], n.b[4], n.b[5], pLo, pHi, inv, eL) && eL <= lim;
+        bool hR = slab_grown(n.b[6], n.b[7], n.b[8], n.b[9], n.b[10], n.b[11], pLo, pHi, inv, eR) && eR <= lim;
+        if (hR) {
+            if (hL) st.push(eL, n.l);
+            cur = n.r;
+            return true;
+        }
+        if (hL) {
+            cur = n.l;
+            return true;
+        }
+        return false;
+    };
+    auto leaf_tests = [&]() {
+        int code = ~cur;
+        int first = code >> 4;
+        int cnt = (code & 15) + 1;
+        if (CNT) w.tests += cnt;
+        for (int k = 0; k < cnt; ++k) {
+            TriRec<float> tr = load_tri(tris, first + k);
+            V3<float> p = cross3(d, tr.e2);
+            float det = dot3(tr.e1, p);
+            float adet = fabsf(det);
+            float invd = 1.0f / det;
+            V3<float> tv = sub(o, tr.v0);
+            float u = dot3(tv, p) * invd;
+            V3<float> q = cross3(tv, tr.e1);
+            float v = dot3(d, q) * invd;
+            float t = dot3(tr.e2, q) * invd;
+            // forward error bounds of this test against the FP64 ray
+            float at = fabsf(t);
+            float v01 = fabsf(tr.v0.x) + fabsf(tr.v0.y) + fabsf(tr.v0.z);
+            float xerr = ex + ed * at + K * kEps32 * (o1 + v01 + 1.0f);
+            // >= 1 / |det|: invd is correctly rounded
+            float rdet = fminf(fabsf(invd), 1e30f) * (1.0f + 4.0f * kEps32);
+            float berr = xerr * tr.sumE * rdet + 4.0f * K * kEps32;
+            float terr = xerr * tr.prodE * rdet + K * kEps32 * (o1 + v01 + at);
+            float ddet = (ed + K * kEps32) * tr.prodE * 2.0f;
+            float mrg = fminf(u, fminf(v, 1.0f - u - v));
+            bool detOk = adet >= 1.0e-6f;
+            bool inside = detOk && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || u + v > 1.0f);
+            bool nearEdge = detOk && fabsf(mrg) <= berr;
+            bool ambig = (fabsf(adet - 1.0e-6f) <= ddet) || nearEdge ||
+                         ((inside || nearEdge) && fabsf(t - 0.001f) <= terr);
+            if (ambig && t >= 0.001f - terr) tAmb = fminf(tAmb, t - terr);
+            // t2: lowest possible true distance of any other hit (each
+            // competitor with its own error bound)
+            if (inside && t >= 0.001f && t <= closest) {
+                if (closest < 1e19f) t2 = fminf(t2, closest - tBestErr);
+                closest = t;
+                tBestErr = terr;
+                hit = first + k;
+            } else if (inside && t > closest) {
+                t2 = fminf(t2, t - terr);
             }
         }
-        bool found = false;
-        while (sp > 0) {
-            --sp;
-            float ee;
-            int rr;
-            if (sp < SD) {
-                ee = sE[sp * kWgSize + tid];
-                rr = sRef[sp * kWgSize + tid];
+    };
+    if constexpr (WW) {
+        for (;;) {
+            while (cur >= 0)
+                if (!node_step()) cur = st.pop(closest + tBestErr + tErr);
+            if (cur == kNoRef) break;
+            leaf_tests();
+            cur = st.pop(closest + tBestErr + tErr);
+            if (cur == kNoRef) break;
+        }
+    } else {
+        // one pop for the lanes that left a leaf and those at a dead end
+        for (;;) {
+            if (cur >= 0) {
+                if (node_step()) continue;
             } else {
-                Ov<float>::unpack(ov[sp - SD], ee, rr);
+                leaf_tests();
             }
-            if (ee <= closest + tBestErr + tErr) {
-                cur = rr;
-                found = true;
-                break;
-            }
+            cur = st.pop(closest + tBestErr + tErr);
+            if (cur == kNoRef) break;
         }
-        if (!found) break;
     }
     if (hit < 0) {
         if (tAmb < 3.0e38f) amb = true;
@@ -552,13 +606,13 @@ template <typename R> __device__ __forceinline__ void ray_init(const Job<R>& J, 
 
 // One query of bounce st.k; returns true when the ray has ended (miss or
 // max_bounces reached). Requires st.k < J.maxB.
-template <typename R, int SD, bool CNT, bool REC>
+template <typename R, int SD, bool CNT, bool REC, bool WW>
 __device__ __forceinline__ bool ray_step(const Job<R>& J, const Mesh<R>& M, RayState<R>& st, int* sRef, R* sE,
                                          typename Ov<R>::T* ov, int tid, int32_t* ids, Work& w,
                                          unsigned long long& queries) {
     V3<R> inv{R(1.0) / st.d.x, R(1.0) / st.d.y, R(1.0) / st.d.z};
     R t;
-    int h = closest_hit<R, SD, CNT>(M, st.o, st.d, inv, t, sRef, sE, ov, tid, w);
+    int h = closest_hit<R, SD, CNT, WW>(M, st.o, st.d, inv, t, sRef, sE, ov, tid, w);
     if (CNT) queries++;
     if (h < 0) return true;
     TriRec<R> tr = load_tri(static_cast<const R*>(M.tris), h);
@@ -577,14 +631,14 @@ __device__ __forceinline__ bool ray_step(const Job<R>& J, const Mesh<R>& M, RayS
 
 // Whole path of one ray in one call. SD == 0 keeps the whole traversal
 // stack in private memory.
-template <typename R, int SD, bool CNT, bool REC>
+template <typename R, int SD, bool CNT, bool REC, bool WW>
 __device__ __forceinline__ PathOut<R> trace_path(const Job<R>& J, const Mesh<R>& M, int i, int j, int* sRef, R* sE,
                                                  typename Ov<R>::T* ov, int tid, int32_t* ids, Work& w,
                                                  unsigned long long& queries) {
     RayState<R> st;
     ray_init(J, i, j, st);
     if (J.maxB > 0)
-        while (!ray_step<R, SD, CNT, REC>(J, M, st, sRef, sE, ov, tid, ids, w, queries)) {
+        while (!ray_step<R, SD, CNT, REC, WW>(J, M, st, sRef, sE, ov, tid, ids, w, queries)) {
         }
     st.P.Ld = double(st.P.L);
     return st.P;
@@ -624,31 +678,51 @@ __device__ __forceinline__ void ray_init_mixed(const Job<double>& D, int i, int 
     st.k = 0;
 }
 
-// One query of bounce st.k: 0 = continue, 1 = ray ended, 2 = the query was
-// ambiguous in FP32 (the ray must be re-traced in FP64).
-template <int SD, bool CNT, bool REC>
-__device__ __forceinline__ int ray_step_mixed(const Job<float>& J, const Mesh<float>& M, RayState<double>& st,
-                                              int* sRef, float* sE, Ov<float>::T* ov, int tid, int32_t* ids,
-                                              Work& w, unsigned long long& queries, float K) {
+// One query of bounce st.k; returns true when the ray has ended (miss or
+// max_bounces reached). A query with an FP32 decision inside its error bound
+// is answered by the reference FP64 query on the FP64 tree instead (from the
+// same FP64 ray state, so it is the reference query itself) and sets fell.
+template <int SD, bool CNT, bool REC, bool WW>
+__device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<float>& M, const Mesh<double>& M64,
+                                               RayState<double>& st, int* sRef, float* sE, Ov<float>::T* ov,
+                                               Ov<double>::T* ov64, int tid, int32_t* ids, Work& w,
+                                               unsigned long long& queries, float K, bool& fell) {
     const V3<double> o = st.o, d = st.d;
     // FP32 copy of the ray; the launch-plane move only shifts the origin
     // along the ray, which changes no decision
     const double shift = st.k == 0 ? J.L0 : 0.0;
     V3<float> of{float(o.x + shift * d.x), float(o.y + shift * d.y), float(o.z + shift * d.z)};
     V3<float> df{float(d.x), float(d.y), float(d.z)};
-    V3<float> inv{1.0f / df.x, 1.0f / df.y, 1.0f / df.z};
+    // finite slab reciprocals: components below 1e-30 are taken as +-1e-30,
+    // a direction change far inside the direction error bound
+    auto away = [](float c) { return copysignf(fmaxf(fabsf(c), 1e-30f), c); };
+    V3<float> inv{1.0f / away(df.x), 1.0f / away(df.y), 1.0f / away(df.z)};
     const float ex = K * kEps32 * (fabsf(of.x) + fabsf(of.y) + fabsf(of.z) + 1e-3f);
     const float ed = 2.0f * K * kEps32;
     float tf;
     bool amb = false;
-    int h = closest_hit_mixed<SD, CNT>(M, of, df, inv, tf, sRef, sE, ov, tid, w, ex, ed, K, amb);
+    int h = closest_hit_mixed<SD, CNT, WW>(M, of, df, inv, tf, sRef, sE, ov, tid, w, ex, ed, K, amb);
     if (CNT) queries++;
-    if (amb) return 2;
-    if (h < 0) return 1;
-    // FP64 recompute of the hit on the chosen triangle (rtKernels.cu:13-41)
+    // FP64 vertices of the chosen triangle (rtKernels.cu:13-41)
     V3<double> v0, e1, e2;
     int face;
-    load_tri_verts64(M.tris64, h, v0, e1, e2, face);
+    if (amb) {
+        fell = true;
+        const V3<double> inv64{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+        double t64;
+        const int h64 = closest_hit<double, 0, CNT, WW>(M64, o, d, inv64, t64, nullptr, nullptr, ov64, tid, w);
+        if (h64 < 0) return true;
+        const TriRec<double> tr = load_tri(static_cast<const double*>(M64.tris), h64);
+        v0 = tr.v0;
+        e1 = tr.e1;
+        e2 = tr.e2;
+        face = tr.face;
+    } else {
+        if (h < 0) return true;
+        load_tri_verts64(M.tris64, h, v0, e1, e2, face);
+    }
+    // FP64 distance and bounce update, as the FP64 query and ray_step form
+    // them (the FP64 tree's edges equal v1 - v0, v2 - v0 of these vertices)
     V3<double> p = cross3(d, e2);
     double det = dot3(e1, p);
     double invd = 1.0 / det;
@@ -665,43 +739,24 @@ __device__ __forceinline__ int ray_step_mixed(const Job<float>& J, const Mesh<fl
     st.d = unit3(axpy(-s, n, d));
     st.o = axpy(1e-4, n, hp);
     st.k++;
-    return st.k >= J.maxB ? 1 : 0;
+    return st.k >= J.maxB;
 }
 
-// Whole mixed-mode path in one call; returns false when a query was ambiguous.
-template <int SD, bool CNT, bool REC>
-__device__ __forceinline__ bool trace_path_mixed(const Job<float>& J, const Job<double>& D, const Mesh<float>& M,
-                                                 int i, int j, int* sRef, float* sE, Ov<float>::T* ov, int tid,
-                                                 int32_t* ids, Work& w, unsigned long long& queries, float K,
-                                                 PathOut<double>& out) {
+// Whole mixed-mode path in one call; fell is set when a query was answered
+// on the FP64 tree.
+template <int SD, bool CNT, bool REC, bool WW>
+__device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J, const Job<double>& D,
+                                                            const Mesh<float>& M, const Mesh<double>& M64, int i,
+                                                            int j, int* sRef, float* sE, Ov<float>::T* ov,
+                                                            Ov<double>::T* ov64, int tid, int32_t* ids, Work& w,
+                                                            unsigned long long& queries, float K, bool& fell) {
     RayState<double> st;
     ray_init_mixed(D, i, j, st);
-    int r = J.maxB > 0 ? 0 : 1;
-    while (r == 0) r = ray_step_mixed<SD, CNT, REC>(J, M, st, sRef, sE, ov, tid, ids, w, queries, K);
+    if (J.maxB > 0)
+        while (!ray_step_mixed<SD, CNT, REC, WW>(J, M, M64, st, sRef, sE, ov, ov64, tid, ids, w, queries, K, fell)) {
+        }
     st.P.Ld = st.P.L;
-    out = st.P;
-    return r != 2;
-}
-
-// Reference FP64 ray with a private stack (mixed-mode re-traces); writes
-// records when ids != nullptr.
-template <bool CNT>
-__device__ __forceinline__ void fp64_retrace(const Job<double>& J, const Mesh<double>& M, int i, int j, int32_t* ids,
-                                             Work& w, unsigned long long& queries, PathOut<double>& P, double& re,
-                                             double& im) {
-    Ov<double>::T ov[kMaxStack];
-    if (ids) {
-        P = trace_path<double, 0, CNT, true>(J, M, i, j, nullptr, nullptr, ov, 0, ids, w, queries);
-        for (int k = P.count; k < J.maxB; ++k) ids[k] = -1;
-    } else {
-        P = trace_path<double, 0, CNT, false>(J, M, i, j, nullptr, nullptr, ov, 0, nullptr, w, queries);
-    }
-    po_term<double>(J, P, false, re, im);
-}
-
-__device__ __forceinline__ void fix_add(unsigned long long* p, double v, double scale) {
-    long long q = __double2ll_rn(v * scale);
-    atomicAdd(p, static_cast<unsigned long long>(q));
+    return st.P;
 }
 
 // Per-ray result slot of the ray-pool variant: the ray's (re, im) term in
@@ -736,7 +791,7 @@ constexpr int kPoolRays = 1024;  // rays per pool batch (4 passes of 16x16)
 template <typename R, int MODE> constexpr int kWavesPerEu =
     MODE == 1 ? SBR_WPE_MIXED : (std::is_same_v<R, double> ? SBR_WPE_FP64 : SBR_WPE_FP32);
 
-template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL>
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW>
 __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<R, MODE>))) void trace_kernel(
     TraceArgs<R> a) {
     using ST = typename SlotOf<R, MODE>::T;
@@ -756,6 +811,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     const int qy = (wave >> 1) * 8 + (lane >> 3);
     unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0, cRetrace = 0;
     typename Ov<R>::T ov[kMaxStack - SD];
+    // mixed mode: private stack of the FP64 queries that answer ambiguous ones
+    Ov<double>::T ov64[MODE == 1 ? kMaxStack : 1];
 
     // Persistent mode pops tiles from a global counter. The pop for the next
     // tile is made by thread 0 in the same block that bumps the job's
@@ -812,6 +869,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 };
                 RayState<RS> st;
                 Work w{0, 0};
+                bool fell = false;
                 bool alive = fetch();
                 auto start = [&]() {
                     if (CNT) cRays++;
@@ -821,46 +879,25 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 if (alive) start();
                 while (__ballot(alive) != 0ull) {
                     if (alive) {
-                        bool done, ok = true;
+                        bool done;
                         if constexpr (MODE == 1) {
-                            const int rc = ray_step_mixed<SD, CNT, false>(J, M, st, sRef, sE, ov, tid, nullptr,
-                                                                          w, cQueries, a.errK);
-                            done = rc != 0;
-                            ok = rc != 2;
+                            done = ray_step_mixed<SD, CNT, false, WW>(J, M, a.meshes64[J.mesh], st, sRef, sE, ov,
+                                                                      ov64, tid, nullptr, w, cQueries, a.errK, fell);
                         } else {
-                            done = ray_step<R, SD, CNT, false>(J, M, st, sRef, sE, ov, tid, nullptr, w,
-                                                               cQueries);
+                            done = ray_step<R, SD, CNT, false, WW>(J, M, st, sRef, sE, ov, tid, nullptr, w,
+                                                                   cQueries);
                         }
                         if (done) {
                             ST re = ST(0), im = ST(0);
                             if constexpr (MODE == 1) {
-                                if (ok) {
-                                    st.P.Ld = st.P.L;
-                                    po_term<double>(a.jobs64[jid], st.P, false, re, im);
-                                    if (CNT) cBounces += st.P.count;
-                                } else {
-                                    if (CNT) cRetrace++;
-                                    const int ray = rj * J.nx + ri;
-                                    unsigned qi = atomicAdd(a.queueCount, 1u);
-                                    if (qi < a.queueCap) {
-                                        a.queue[qi] = QEntry{jid, ray};
-                                    } else {
-                                        // queue full: re-trace inline (same FP64 path, same fixed-point sum)
-                                        const Job<double>& J64 = a.jobs64[jid];
-                                        PathOut<double> P64;
-                                        double r64, i64;
-                                        Work w2{0, 0};
-                                        unsigned long long q2 = 0;
-                                        fp64_retrace<false>(J64, a.meshes64[J64.mesh], ri, rj, nullptr, w2, q2, P64,
-                                                            r64, i64);
-                                        fix_add(&a.jobFix[2 * jid], r64, J64.fixScale);
-                                        fix_add(&a.jobFix[2 * jid + 1], i64, J64.fixScale);
-                                    }
-                                }
+                                st.P.Ld = st.P.L;
+                                po_term<double>(a.jobs64[jid], st.P, false, re, im);
+                                if (CNT && fell) cRetrace++;
+                                fell = false;
                             } else {
                                 po_term<R>(J, st.P, false, re, im);
-                                if (CNT) cBounces += st.P.count;
                             }
+                            if (CNT) cBounces += st.P.count;
                             sRes[2 * r] = re;
                             sRes[2 * r + 1] = im;
                             alive = fetch();
@@ -904,17 +941,16 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     R re = 0, im = 0;
                     double dre = 0.0, dimg = 0.0;
                     PathOut<R> P;
-                    PathOut<double> PD;
-                    bool ok = true;
+                    bool fell = false;
                     if constexpr (MODE == 1) {
                         const Job<double>& D = a.jobs64[jid];
-                        ok = trace_path_mixed<SD, CNT, REC>(J, D, M, i, j, sRef, sE, ov, tid, ids, w, cQueries,
-                                                           a.errK, PD);
-                        if (ok) po_term<double>(D, PD, false, dre, dimg);
+                        const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW>(
+                            J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, tid, ids, w, cQueries, a.errK, fell);
+                        po_term<double>(D, PD, false, dre, dimg);
                         P.count = PD.count;
                         P.Ld = PD.Ld;
                     } else {
-                        P = trace_path<R, SD, CNT, REC>(J, M, i, j, sRef, sE, ov, tid, ids, w, cQueries);
+                        P = trace_path<R, SD, CNT, REC, WW>(J, M, i, j, sRef, sE, ov, tid, ids, w, cQueries);
                         po_term<R>(J, P, false, re, im);
                     }
                     if (CNT) {
@@ -922,32 +958,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                         cTests += w.tests;
                     }
                     if constexpr (MODE == 1) {
-                        if (!ok) {
-                            if (CNT) cRetrace++;
-                            if (REC) a.recFlag[ray] = 1;
-                            unsigned q = atomicAdd(a.queueCount, 1u);
-                            if (q < a.queueCap) {
-                                a.queue[q] = QEntry{jid, ray};
-                            } else {
-                                // queue full: re-trace inline (same FP64 path, same fixed-point sum)
-                                const Job<double>& J64 = a.jobs64[jid];
-                                PathOut<double> P64;
-                                double r64, i64;
-                                Work w2{0, 0};
-                                unsigned long long q2 = 0;
-                                fp64_retrace<false>(J64, a.meshes64[J64.mesh], i, j, REC ? ids : nullptr, w2, q2, P64,
-                                                    r64, i64);
-                                fix_add(&a.jobFix[2 * jid], r64, J64.fixScale);
-                                fix_add(&a.jobFix[2 * jid + 1], i64, J64.fixScale);
-                                if (REC) {
-                                    a.recCount[ray] = P64.count;
-                                    a.recL[ray] = P64.Ld;
-                                    a.recReIm[2 * size_t(ray)] = r64;
-                                    a.recReIm[2 * size_t(ray) + 1] = i64;
-                                }
-                            }
-                            break;
-                        }
+                        if (CNT && fell) cRetrace++;
+                        if (REC) a.recFlag[ray] = fell ? 1 : 0;
                     }
                     if (CNT) cBounces += P.count;
                     if (REC) {
@@ -997,54 +1009,6 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
             if (lane == 0) atomicAdd(&a.counters[q], v);
         }
     }
-}
-
-// FP64 re-traces of the mixed mode's flagged rays (grid-stride over the queue).
-template <bool REC, bool CNT>
-__global__ __launch_bounds__(kWgSize) void retrace_kernel(TraceArgs<float> a) {
-    const unsigned n = min(*a.queueCount, a.queueCap);
-    unsigned long long cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0;
-    for (unsigned idx = blockIdx.x * kWgSize + threadIdx.x; idx < n; idx += gridDim.x * kWgSize) {
-        QEntry e = a.queue[idx];
-        const Job<double>& J = a.jobs64[e.job];
-        const int i = e.ray % J.nx, j = e.ray / J.nx;
-        Work w{0, 0};
-        PathOut<double> P;
-        double re, im;
-        fp64_retrace<CNT>(J, a.meshes64[J.mesh], i, j, REC ? a.recIds + size_t(e.ray) * J.maxB : nullptr, w,
-                          cQueries, P, re, im);
-        fix_add(&a.jobFix[2 * e.job], re, J.fixScale);
-        fix_add(&a.jobFix[2 * e.job + 1], im, J.fixScale);
-        if (CNT) {
-            cVisits += w.visits;
-            cTests += w.tests;
-            cBounces += P.count;
-        }
-        if (REC) {
-            a.recCount[e.ray] = P.count;
-            a.recL[e.ray] = P.Ld;
-            a.recReIm[2 * size_t(e.ray)] = re;
-            a.recReIm[2 * size_t(e.ray) + 1] = im;
-        }
-    }
-    if (CNT) {
-        unsigned long long c[4] = {cVisits, cTests, cBounces, cQueries};
-        const int lane = threadIdx.x & 63;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            unsigned long long v = wave_sum(c[q]);
-            if (lane == 0) atomicAdd(&a.counters[6 + q], v);
-        }
-    }
-}
-
-// Adds the fixed-point FP64 re-trace sums to the job outputs.
-__global__ void finalize_kernel(const Job<double>* jobs64, int n, const unsigned long long* fix, double* out) {
-    int jid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (jid >= n) return;
-    const double inv = 1.0 / jobs64[jid].fixScale;
-    out[2 * jid] += double(static_cast<long long>(fix[2 * jid])) * inv;
-    out[2 * jid + 1] += double(static_cast<long long>(fix[2 * jid + 1])) * inv;
 }
 
 __global__ void zero_words(unsigned* p, size_t n) {
@@ -1203,67 +1167,61 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     if (threadIdx.x == 0) atomicAdd(a.dfreq, red[0] + red[1] + red[2] + red[3]);
 }
 
-template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL>
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW>
 hipError_t launch_one(const TraceArgs<R>& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST, POOL>), dim3(grid), dim3(kWgSize), 0, s, a);
+    hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST, POOL, WW>), dim3(grid), dim3(kWgSize), 0, s, a);
     return hipGetLastError();
 }
 
+// Record and counting runs use the while-while shape only (their outputs
+// do not depend on the shape).
 template <typename R, int MODE, int SD, bool POOL>
-hipError_t launch_pool(const TraceArgs<R>& a, bool cnt, bool persist, int grid, hipStream_t s) {
-    if (cnt) return persist ? launch_one<R, MODE, SD, false, true, true, POOL>(a, grid, s)
-                            : launch_one<R, MODE, SD, false, true, false, POOL>(a, grid, s);
-    return persist ? launch_one<R, MODE, SD, false, false, true, POOL>(a, grid, s)
-                   : launch_one<R, MODE, SD, false, false, false, POOL>(a, grid, s);
+hipError_t launch_pool(const TraceArgs<R>& a, bool cnt, bool persist, bool ww, int grid, hipStream_t s) {
+    if (cnt) return persist ? launch_one<R, MODE, SD, false, true, true, POOL, true>(a, grid, s)
+                            : launch_one<R, MODE, SD, false, true, false, POOL, true>(a, grid, s);
+    if (ww)
+        return persist ? launch_one<R, MODE, SD, false, false, true, POOL, true>(a, grid, s)
+                       : launch_one<R, MODE, SD, false, false, false, POOL, true>(a, grid, s);
+    return persist ? launch_one<R, MODE, SD, false, false, true, POOL, false>(a, grid, s)
+                   : launch_one<R, MODE, SD, false, false, false, POOL, false>(a, grid, s);
 }
 
 template <typename R, int MODE, int SD>
-hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, bool pool, int grid, hipStream_t s) {
-    if (rec) return launch_one<R, MODE, SD, true, false, false, false>(a, grid, s);
-    return pool ? launch_pool<R, MODE, SD, true>(a, cnt, persist, grid, s)
-                : launch_pool<R, MODE, SD, false>(a, cnt, persist, grid, s);
+hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, bool pool, bool ww, int grid,
+                     hipStream_t s) {
+    if (rec) return launch_one<R, MODE, SD, true, false, false, false, true>(a, grid, s);
+    return pool ? launch_pool<R, MODE, SD, true>(a, cnt, persist, ww, grid, s)
+                : launch_pool<R, MODE, SD, false>(a, cnt, persist, ww, grid, s);
 }
 
 }  // namespace
 
 // Host entry used by the driver. sd: traversal stack entries kept in LDS
-// per ray (8, 16 or 32); deeper entries spill to scratch.
+// per ray (8, 16 or 32); deeper entries spill to scratch. ww: while-while
+// traversal loop (else if-if).
 template <typename R>
-hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool,
+hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool, bool ww,
                         int grid, hipStream_t s) {
     if constexpr (std::is_same_v<R, float>) {
         if (mode == 1) {
-            if (sd <= 8) return launch_sd<float, 1, 8>(a, rec, cnt, persist, pool, grid, s);
-            if (sd <= 16) return launch_sd<float, 1, 16>(a, rec, cnt, persist, pool, grid, s);
-            return launch_sd<float, 1, 32>(a, rec, cnt, persist, pool, grid, s);
+            if (sd <= 8) return launch_sd<float, 1, 8>(a, rec, cnt, persist, pool, ww, grid, s);
+            if (sd <= 16) return launch_sd<float, 1, 16>(a, rec, cnt, persist, pool, ww, grid, s);
+            return launch_sd<float, 1, 32>(a, rec, cnt, persist, pool, ww, grid, s);
         }
     }
-    if (sd <= 8) return launch_sd<R, 0, 8>(a, rec, cnt, persist, pool, grid, s);
-    if (sd <= 16) return launch_sd<R, 0, 16>(a, rec, cnt, persist, pool, grid, s);
-    return launch_sd<R, 0, 32>(a, rec, cnt, persist, pool, grid, s);
+    if (sd <= 8) return launch_sd<R, 0, 8>(a, rec, cnt, persist, pool, ww, grid, s);
+    if (sd <= 16) return launch_sd<R, 0, 16>(a, rec, cnt, persist, pool, ww, grid, s);
+    return launch_sd<R, 0, 32>(a, rec, cnt, persist, pool, ww, grid, s);
 }
 
-template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool, bool, bool, bool, int, hipStream_t);
-template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, bool, int,
+template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool, bool, bool, bool, bool, int,
+                                        hipStream_t);
+template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, bool, bool, int,
                                          hipStream_t);
-
-hipError_t launch_retrace(const TraceArgs<float>& a, bool rec, bool cnt, int grid, hipStream_t s) {
-    if (rec) hipLaunchKernelGGL((retrace_kernel<true, false>), dim3(grid), dim3(kWgSize), 0, s, a);
-    else if (cnt) hipLaunchKernelGGL((retrace_kernel<false, true>), dim3(grid), dim3(kWgSize), 0, s, a);
-    else hipLaunchKernelGGL((retrace_kernel<false, false>), dim3(grid), dim3(kWgSize), 0, s, a);
-    return hipGetLastError();
-}
 
 hipError_t launch_adjoint(const AdjArgs& a, hipStream_t s) {
     if (a.nRays <= 0) return hipSuccess;
     hipLaunchKernelGGL(adjoint_kernel, dim3((a.nRays + 255) / 256), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_finalize(const Job<double>* jobs64, int n, const unsigned long long* fix, double* out,
-                           hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, s, jobs64, n, fix, out);
     return hipGetLastError();
 }
 
@@ -1276,8 +1234,8 @@ hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s) {
 
 template <typename R, int MODE, int SD> int occ_one(bool pool) {
     int n = 0;
-    const void* f = pool ? reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, true>)
-                         : reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, false>);
+    const void* f = pool ? reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, true, true>)
+                         : reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, false, true>);
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, kWgSize, 0);
     return e == hipSuccess ? n : 1;
 }

@@ -96,12 +96,14 @@ def test_per_ray_fp64_bitexact_vs_oracle(ctx, sphere, tri_duct, case, angles):
 
 @pytest.mark.parametrize("case,angles", [("sphere", [0, 37]), ("tri_duct", [0, 4140, 2790, 8280, 7188, 4161])])
 def test_per_ray_mixed_has_fp64_outcome(ctx, sphere, tri_duct, case, angles):
-    """Mixed mode: every ray ends with the FP64 oracle's outcome -- FP32 rays
-    because none of their decisions was within its error bound, flagged rays
-    because they were re-traced in FP64 (bit-exact with the oracle)."""
+    """Mixed mode: every ray ends with the FP64 oracle's outcome -- queries
+    decided in FP32 because none of their decisions was within its error
+    bound, the others because they were answered by the FP64 query on the
+    FP64 tree (flagged rays). The trihedral+duct angles include such rays."""
     v, f, mesh, s32, s64 = sphere if case == "sphere" else tri_duct
     kw = C1 if case == "sphere" else C2S
     p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    flagged = 0
     for a in angles:
         print(f"[per-ray mixed] {case} angle {a}", flush=True)
         g = ctx.trace_rays(mesh, p, a, S.SBR_MIXED)
@@ -113,6 +115,10 @@ def test_per_ray_mixed_has_fp64_outcome(ctx, sphere, tri_duct, case, angles):
         np.testing.assert_allclose(g["reim"], o["reim"], rtol=0, atol=1e-12 * np.abs(o["reim"]).max())
         rt = g["retraced"] == 1
         assert rt.mean() < 0.05
+        flagged += int(rt.sum())
+    print(f"    rays with an FP64-answered query: {flagged}", flush=True)
+    if case == "tri_duct":
+        assert flagged > 0, "the FP64 answer path was not exercised"
 
 
 def test_sphere_sigma_all_precisions(ctx, sphere):
@@ -222,7 +228,8 @@ def test_graph_replay_matches_eager(ctx, sphere, tri_duct):
 @pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_MIXED, S.SBR_FP64])
 def test_tuning_variants(ctx, tri_duct, prec):
     """Partial sums are kept per 16x16 ray pass and reduced in a fixed order,
-    so every launch shape and both methods give the same sigma to the bit."""
+    so every launch shape, traversal schedule and both methods give the same
+    sigma to the bit."""
     v, f, mesh, s32, s64 = tri_duct
     p = S.SweepParams.make(**dict(C2S, theta_samples=5, phi_samples=5))
     jobs = S.sweep_jobs(p, mesh, prec)
@@ -240,16 +247,26 @@ def test_tuning_variants(ctx, tri_duct, prec):
                     pl.wait()
                     z, _, _ = pl.results()
                     assert np.array_equal(z, z0), (pool, ts, sd, wg)
+    for trav in (1, 2):
+        for pool in (1, 2):
+            for ts in (16, 64):
+                _step(f"precision {prec} batched traversal={trav} pool={pool} tile={ts}")
+                pl.tune(traversal=trav, ray_pool=pool, tile_rays=ts)
+                pl.run()
+                pl.wait()
+                z, _, _ = pl.results()
+                assert np.array_equal(z, z0), (trav, pool, ts)
     pl.close()
     pa = ctx.plan(jobs, prec, S.SBR_PER_ANGLE)
     for pool in (1, 2):
         for ts in (16, 64):
-            _step(f"precision {prec} per-angle pool={pool} tile={ts}")
-            pa.tune(tile_rays=ts, ray_pool=pool)
+            trav = 1 if ts == 16 else 2
+            _step(f"precision {prec} per-angle pool={pool} tile={ts} traversal={trav}")
+            pa.tune(tile_rays=ts, ray_pool=pool, traversal=trav)
             pa.run()
             pa.wait()
             z, _, _ = pa.results()
-            assert np.array_equal(z, z0), (pool, ts)
+            assert np.array_equal(z, z0), (pool, ts, trav)
     pa.close()
 
 
@@ -265,9 +282,9 @@ def test_autotune(ctx, tri_duct):
         _step(f"autotune method {method}")
         best = pl.autotune(sample_jobs=8 if method == S.SBR_BATCHED else 0, repeats=1)
         print("   ", best, flush=True)
-        assert best["candidates"] == (36 if method == S.SBR_BATCHED else 24)
+        assert best["candidates"] == (36 if method == S.SBR_BATCHED else 24) + 1
         assert best["tile_rays"] in (16, 32, 64) and best["lds_stack"] in (8, 16, 32) and best["best_ms"] > 0
-        assert best["ray_pool"] in (1, 2)
+        assert best["ray_pool"] in (1, 2) and best["traversal"] in (0, 1)
         pl.run()
         pl.wait()
         z, _, _ = pl.results()
@@ -279,9 +296,9 @@ def test_autotune(ctx, tri_duct):
 def test_mixed_airframe_fin_angles(ctx, theta, phi):
     """Regression: on the airframe's fin two triangles are hit 8e-5 m apart
     and the farther one has a tiny determinant (distance error ~5e-3 m).
-    The mixed mode must flag such a ray (the competitor's own error bound
-    counts) and end with the FP64 outcome: per-ray bit-exact against the
-    FP64 path (itself bit-exact with the FP64 oracle)."""
+    The mixed mode must not decide such a query in FP32 (the competitor's
+    own error bound counts) and must end with the FP64 outcome: per-ray
+    bit-exact against the FP64 path (itself bit-exact with the FP64 oracle)."""
     from sagittasbr_amd import workloads as W
 
     wl = W.c3((3e9,))
@@ -298,7 +315,7 @@ def test_mixed_airframe_fin_angles(ctx, theta, phi):
     assert np.array_equal(gm["count"], g6["count"])
     assert np.array_equal(gm["ids"], g6["ids"])
     assert np.array_equal(gm["L"], g6["L"])
-    print(f"    rays {len(gm['count'])}, re-traced {int(gm['retraced'].sum())}", flush=True)
+    print(f"    rays {len(gm['count'])}, with an FP64-answered query {int(gm['retraced'].sum())}", flush=True)
 
 
 def test_plan_outliving_context_fails_cleanly():
