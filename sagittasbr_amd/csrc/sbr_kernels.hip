@@ -153,6 +153,16 @@ __device__ __forceinline__ void load_tri_verts64(const void* tris64, int i, V3<d
     e2 = {d.x - v0.x, d.y - v0.y, e.x - v0.z};
     face = static_cast<int>(__double_as_longlong(e.y));
 }
+// The same record's three vertices as stored, and the face id.
+__device__ __forceinline__ void load_tri_raw64(const void* tris64, int i, V3<double>& v0, V3<double>& v1,
+                                               V3<double>& v2, int& face) {
+    gptr<vd2> p = as_global<vd2>(tris64) + 5 * i;
+    vd2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    v0 = {a.x, a.y, b.x};
+    v1 = {b.y, c.x, c.y};
+    v2 = {d.x, d.y, e.x};
+    face = static_cast<int>(__double_as_longlong(e.y));
+}
 __device__ __forceinline__ TriRec<double> load_tri(const double* tris, int i) {
     gptr<vd2> p = as_global<vd2>(tris) + 5 * i;
     vd2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
@@ -354,11 +364,55 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
 
 // ------------------------------------------------- mixed-mode traversal
 constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
+constexpr int kCand = 4;                 // candidate slots per lane
 
+// Candidate triangles of an undecided mixed query, kept per lane in LDS at
+// [slot * kWgSize + tid]: every certified hit that may tie the best one and
+// every triangle whose FP64 outcome is open (with the lowest distance it can
+// have), as long as it may beat or tie the best certified hit.
+struct Cands {
+    int* ids;
+    float* lbs;
+    int tid;
+    int n;
+    bool over;  // a relevant candidate found no slot
+    // drop the entries beyond lim, then append (idx, lb)
+    __device__ __forceinline__ void add(int idx, float lb, float lim) {
+        if (n == kCand) compact(lim);
+        if (n < kCand) {
+            ids[n * kWgSize + tid] = idx;
+            lbs[n * kWgSize + tid] = lb;
+            ++n;
+        } else {
+            over = true;
+        }
+    }
+    __device__ __forceinline__ void compact(float lim) {
+        int m = 0;
+        for (int q = 0; q < n; ++q) {
+            const float l = lbs[q * kWgSize + tid];
+            if (l <= lim) {
+                if (m != q) {
+                    ids[m * kWgSize + tid] = ids[q * kWgSize + tid];
+                    lbs[m * kWgSize + tid] = l;
+                }
+                ++m;
+            }
+        }
+        n = m;
+    }
+};
+
+// FP32 closest-hit query with error-bounded decisions. Returns the FP32-tree
+// position of the best certified hit (an FP64 hit too, at tHit +- its bound)
+// or -1; amb is set when some candidate may beat or tie it (the query is then
+// undecided in FP32 and the candidates are left in cand). Pruning uses the
+// best certified hit only, so every triangle that could be the FP64 answer
+// is tested.
 template <int SD, bool CNT, bool WW>
 __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> d, V3<float> inv,
                                                  float& tHit, int* sRef, float* sE, Ov<float>::T* ov, int tid,
-                                                 Work& w, float ex, float ed, float K, bool& amb) {
+                                                 Work& w, float ex, float ed, float K, Cands& cand, bool& amb) {
     const float* nodes = static_cast<const float*>(m.nodes);
     const float* tris = static_cast<const float*>(m.tris);
     const float o1 = fabsf(o.x) + fabsf(o.y) + fabsf(o.z);
@@ -372,8 +426,9 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     float dx = ex + ed * tFar + K * kEps32 * (o1 + tFar);
     float tErr = dx + 4.0f * K * kEps32 * tFar;
     float closest = 1e20f, tBestErr = 0.0f;
-    float tAmb = 3.0e38f, t2 = 3.0e38f;
     int hit = -1;
+    // lowest distance a rival may have and still beat or tie the best
+    auto rival_lim = [&]() { return closest + 2.0f * tBestErr + 1.0e-6f * closest; };
     float e;
     if (CNT) w.visits += 1;
     // o + dx and o - dx rounded outwards: the margin covers the rounding of
@@ -439,16 +494,19 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
             bool nearEdge = detOk && fabsf(mrg) <= berr;
             bool ambig = (fabsf(adet - 1.0e-6f) <= ddet) || nearEdge ||
                          ((inside || nearEdge) && fabsf(t - 0.001f) <= terr);
-            if (ambig && t >= 0.001f - terr) tAmb = fminf(tAmb, t - terr);
-            // t2: lowest possible true distance of any other hit (each
-            // competitor with its own error bound)
-            if (inside && t >= 0.001f && t <= closest) {
-                if (closest < 1e19f) t2 = fminf(t2, closest - tBestErr);
+            // certified: an FP64 hit at t +- terr; open: may or may not be
+            // an FP64 hit (at t +- terr if it is)
+            const bool cert = inside && !ambig && t >= 0.001f;
+            const bool open = ambig && t >= 0.001f - terr;
+            if (cert && t <= closest) {
+                // the replaced best stays a rival while it may tie the new one
+                if (hit >= 0 && closest - tBestErr <= t + 2.0f * terr + 1.0e-6f * t)
+                    cand.add(hit, closest - tBestErr, t + 2.0f * terr + 1.0e-6f * t);
                 closest = t;
                 tBestErr = terr;
                 hit = first + k;
-            } else if (inside && t > closest) {
-                t2 = fminf(t2, t - terr);
+            } else if ((cert || open) && t - terr <= rival_lim()) {
+                cand.add(first + k, t - terr, rival_lim());
             }
         }
     };
@@ -473,13 +531,9 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
             if (cur == kNoRef) break;
         }
     }
-    if (hit < 0) {
-        if (tAmb < 3.0e38f) amb = true;
-    } else {
-        float tol = 2.0f * tBestErr + 1.0e-6f * closest;
-        if (tAmb <= closest + tol) amb = true;
-        if (t2 <= closest + tol) amb = true;
-    }
+    // the final best decides which candidates still matter
+    cand.compact(rival_lim());
+    amb = cand.over || cand.n > 0;
     tHit = closest;
     return hit;
 }
@@ -678,15 +732,72 @@ __device__ __forceinline__ void ray_init_mixed(const Job<double>& D, int i, int 
     st.k = 0;
 }
 
+// FP64 answer of an undecided query from its candidates: the reference FP64
+// Moller-Trumbore test (rtKernels.cu:10-44) of each candidate and of the
+// certified best h. The reference query returns the nearest accepted
+// triangle, so that is the answer when (a) no other accepted candidate is
+// within 1e-12 of its distance (the reference breaks exact ties by test
+// order, which the FP32 tree does not share) and (b) its hit point lies in
+// the triangle's box with a margin far above FP64 rounding on every axis the
+// triangle spans, so the reference's box tests on the way to it cannot
+// reject it by rounding (on an axis the triangle does not span, both slab
+// planes of a leaf box give the same value). Every triangle that could be
+// the answer is among the candidates (pruning used certified hits only).
+// Rounding of the reference's FP64 slab and distance arithmetic stays below
+// 2^-51 (|o| + |P| + R) in position units (R: the mesh's largest coordinate
+// magnitude per axis, summed), so ties and margins are judged against 1e-12
+// of that. Returns the FP32-tree position of the answer, -1 for a miss, or
+// -3 (near tie), -4 (margin), -5 (the certified hit rejected) when the full
+// FP64 query must decide.
+__device__ __forceinline__ int resolve_fp64(const void* tris64, V3<double> o, V3<double> d, int h, const Cands& c,
+                                            double scale) {
+    double best = 3.0e300, second = 3.0e300;
+    int win = -1;
+    auto test = [&](int idx) {
+        V3<double> a, b, e;
+        int f;
+        load_tri_raw64(tris64, idx, a, b, e, f);
+        const TriRec<double> tr{a, sub(b, a), sub(e, a), f, 0.f, 0.f};
+        double t;
+        if (mt_test(tr, o, d, 1e20, t)) {
+            if (t < best) {
+                second = best;
+                best = t;
+                win = idx;
+            } else if (t < second) {
+                second = t;
+            }
+        }
+    };
+    if (h >= 0) test(h);
+    for (int q = 0; q < c.n; ++q) test(c.ids[q * kWgSize + c.tid]);
+    if (win < 0) return h >= 0 ? -5 : -1;
+    const V3<double> P = axpy(best, d, o);
+    const double margin = 1e-12 * (scale + fabs(o.x) + fabs(o.y) + fabs(o.z) + fabs(P.x) + fabs(P.y) + fabs(P.z));
+    if (second - best <= margin) return -3;
+    V3<double> a, b, e;
+    int f;
+    load_tri_raw64(tris64, win, a, b, e, f);
+    auto inside_axis = [&](double pa, double x0, double x1, double x2) {
+        const double lo = fmin(x0, fmin(x1, x2)), hi = fmax(x0, fmax(x1, x2));
+        return !(hi > lo) || (pa - lo > margin && hi - pa > margin);
+    };
+    if (!inside_axis(P.x, a.x, b.x, e.x) || !inside_axis(P.y, a.y, b.y, e.y) || !inside_axis(P.z, a.z, b.z, e.z))
+        return -4;
+    return win;
+}
+
 // One query of bounce st.k; returns true when the ray has ended (miss or
 // max_bounces reached). A query with an FP32 decision inside its error bound
-// is answered by the reference FP64 query on the FP64 tree instead (from the
-// same FP64 ray state, so it is the reference query itself) and sets fell.
+// (undecided) is answered in FP64 and sets fell: from its candidates when
+// local is set and they decide it, otherwise by the reference FP64 query on
+// the FP64 tree from the same FP64 ray state (the reference query itself).
 template <int SD, bool CNT, bool REC, bool WW>
 __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<float>& M, const Mesh<double>& M64,
                                                RayState<double>& st, int* sRef, float* sE, Ov<float>::T* ov,
-                                               Ov<double>::T* ov64, int tid, int32_t* ids, Work& w,
-                                               unsigned long long& queries, float K, bool& fell) {
+                                               Ov<double>::T* ov64, int* cIds, float* cLbs, int tid, int32_t* ids,
+                                               Work& w, unsigned long long& queries, unsigned long long& fullQ,
+                                               float K, bool local, bool& fell) {
     const V3<double> o = st.o, d = st.d;
     // FP32 copy of the ray; the launch-plane move only shifts the origin
     // along the ray, which changes no decision
@@ -701,13 +812,28 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
     const float ed = 2.0f * K * kEps32;
     float tf;
     bool amb = false;
-    int h = closest_hit_mixed<SD, CNT, WW>(M, of, df, inv, tf, sRef, sE, ov, tid, w, ex, ed, K, amb);
+    Cands cand{cIds, cLbs, tid, 0, false};
+    int h = closest_hit_mixed<SD, CNT, WW>(M, of, df, inv, tf, sRef, sE, ov, tid, w, ex, ed, K, cand, amb);
     if (CNT) queries++;
     // FP64 vertices of the chosen triangle (rtKernels.cu:13-41)
     V3<double> v0, e1, e2;
     int face;
+    int win = -2;
     if (amb) {
         fell = true;
+        if (local && !cand.over) {
+            const double R = 1.0 + fmax(fabs(M.rootLo[0]), fabs(M.rootHi[0])) +
+                             fmax(fabs(M.rootLo[1]), fabs(M.rootHi[1])) + fmax(fabs(M.rootLo[2]), fabs(M.rootHi[2]));
+            win = resolve_fp64(M.tris64, o, d, h, cand, R);
+        }
+        if (win == -1) return true;
+    }
+    if (win >= 0 || (!amb && h >= 0)) {
+        load_tri_verts64(M.tris64, win >= 0 ? win : h, v0, e1, e2, face);
+    } else if (!amb) {
+        return true;
+    } else {
+        if (CNT) fullQ++;
         const V3<double> inv64{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
         double t64;
         const int h64 = closest_hit<double, 0, CNT, WW>(M64, o, d, inv64, t64, nullptr, nullptr, ov64, tid, w);
@@ -717,9 +843,6 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
         e1 = tr.e1;
         e2 = tr.e2;
         face = tr.face;
-    } else {
-        if (h < 0) return true;
-        load_tri_verts64(M.tris64, h, v0, e1, e2, face);
     }
     // FP64 distance and bounce update, as the FP64 query and ray_step form
     // them (the FP64 tree's edges equal v1 - v0, v2 - v0 of these vertices)
@@ -748,12 +871,15 @@ template <int SD, bool CNT, bool REC, bool WW>
 __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J, const Job<double>& D,
                                                             const Mesh<float>& M, const Mesh<double>& M64, int i,
                                                             int j, int* sRef, float* sE, Ov<float>::T* ov,
-                                                            Ov<double>::T* ov64, int tid, int32_t* ids, Work& w,
-                                                            unsigned long long& queries, float K, bool& fell) {
+                                                            Ov<double>::T* ov64, int* cIds, float* cLbs, int tid,
+                                                            int32_t* ids, Work& w, unsigned long long& queries,
+                                                            unsigned long long& fullQ, float K, bool local,
+                                                            bool& fell) {
     RayState<double> st;
     ray_init_mixed(D, i, j, st);
     if (J.maxB > 0)
-        while (!ray_step_mixed<SD, CNT, REC, WW>(J, M, M64, st, sRef, sE, ov, ov64, tid, ids, w, queries, K, fell)) {
+        while (!ray_step_mixed<SD, CNT, REC, WW>(J, M, M64, st, sRef, sE, ov, ov64, cIds, cLbs, tid, ids, w, queries,
+                                                 fullQ, K, local, fell)) {
         }
     st.P.Ld = st.P.L;
     return st.P;
@@ -800,6 +926,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     __shared__ R sE[SD * kWgSize];
     __shared__ Acc<R, MODE> red[kWgSize / 64];
     __shared__ ST sRes[POOL ? 2 * kPoolRays : 1];
+    __shared__ int sCi[MODE == 1 ? kCand * kWgSize : 1];     // mixed: candidate triangles
+    __shared__ float sCl[MODE == 1 ? kCand * kWgSize : 1];   // and their lowest distances
     __shared__ int64_t sNext;
     __shared__ unsigned sPrev;
     __shared__ unsigned sPool;
@@ -809,7 +937,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     // wave -> 8x8 quadrant of a 16x16 ray pass, lane -> ray within the quadrant
     const int qx = (wave & 1) * 8 + (lane & 7);
     const int qy = (wave >> 1) * 8 + (lane >> 3);
-    unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0, cRetrace = 0;
+    unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0, cRetrace = 0, cFull = 0;
     typename Ov<R>::T ov[kMaxStack - SD];
     // mixed mode: private stack of the FP64 queries that answer ambiguous ones
     Ov<double>::T ov64[MODE == 1 ? kMaxStack : 1];
@@ -881,8 +1009,9 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     if (alive) {
                         bool done;
                         if constexpr (MODE == 1) {
-                            done = ray_step_mixed<SD, CNT, false, WW>(J, M, a.meshes64[J.mesh], st, sRef, sE, ov,
-                                                                      ov64, tid, nullptr, w, cQueries, a.errK, fell);
+                            done = ray_step_mixed<SD, CNT, false, WW>(
+                                J, M, a.meshes64[J.mesh], st, sRef, sE, ov, ov64, sCi, sCl, tid, nullptr, w, cQueries,
+                                cFull, a.errK, a.localFp64 != 0, fell);
                         } else {
                             done = ray_step<R, SD, CNT, false, WW>(J, M, st, sRef, sE, ov, tid, nullptr, w,
                                                                    cQueries);
@@ -945,7 +1074,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     if constexpr (MODE == 1) {
                         const Job<double>& D = a.jobs64[jid];
                         const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW>(
-                            J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, tid, ids, w, cQueries, a.errK, fell);
+                            J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, sCi, sCl, tid, ids, w, cQueries,
+                            cFull, a.errK, a.localFp64 != 0, fell);
                         po_term<double>(D, PD, false, dre, dimg);
                         P.count = PD.count;
                         P.Ld = PD.Ld;
@@ -1002,9 +1132,9 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     }
 
     if (CNT) {
-        unsigned long long c[6] = {cRays, cVisits, cTests, cBounces, cQueries, cRetrace};
+        unsigned long long c[7] = {cRays, cVisits, cTests, cBounces, cQueries, cRetrace, cFull};
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
+        for (int q = 0; q < 7; ++q) {
             unsigned long long v = wave_sum(c[q]);
             if (lane == 0) atomicAdd(&a.counters[q], v);
         }

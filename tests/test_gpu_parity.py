@@ -161,6 +161,49 @@ def test_tri_duct_sigma_mixed_vs_fp64(ctx, tri_duct):
     assert np.array_equal(zm, zp)
 
 
+@pytest.mark.parametrize("scene", ["tri_duct", "airframe"])
+def test_mixed_fp64_answer_modes_agree(ctx, tri_duct, scene):
+    """Undecided mixed queries answered from their candidates (default) or
+    always by the full FP64 query: every ray has the FP64 outcome either way,
+    so the sums are identical to the bit, and both modes must meet undecided
+    queries. On the airframe the candidates decide most of them; the
+    trihedral+duct at these axis-aligned angles sends rays along plate planes
+    and seams, whose near-ties and on-boundary hits only the full query can
+    order as the reference does."""
+    if scene == "tri_duct":
+        v, f, mesh, s32, s64 = tri_duct
+        p = S.SweepParams.make(**dict(C2S, theta_samples=6, phi_samples=6))
+    else:
+        from sagittasbr_amd import workloads as W
+
+        wl = W.c3((3e9,))
+        v, f = wl["meshes"][0]
+        mesh = ctx.mesh(v, f)
+        p = S.SweepParams.make(**dict(wl["sweeps"][0][1], theta_start=88, theta_end=92, theta_samples=3,
+                                      phi_start=223, phi_end=243, phi_samples=3))
+    jobs = S.sweep_jobs(p, mesh, S.SBR_MIXED)
+    out = {}
+    for mode in (1, 2):
+        _step(f"{scene} mixed_fp64={mode}")
+        pl = ctx.plan(jobs, S.SBR_MIXED, S.SBR_BATCHED)
+        pl.tune(mixed_fp64=mode)
+        pl.run()
+        pl.wait()
+        z, _, _ = pl.results()
+        c = pl.count()
+        pl.close()
+        out[mode] = (z, c)
+        print(f"    rays {c['rays']}, with an undecided query {c['retraced']}, full FP64 queries "
+              f"{c['fp64_queries']}", flush=True)
+    assert np.array_equal(out[1][0], out[2][0])
+    assert out[1][1]["retraced"] == out[2][1]["retraced"] > 0
+    assert out[2][1]["fp64_queries"] >= out[2][1]["retraced"]
+    frac = 0.2 if scene == "airframe" else 0.95
+    assert out[1][1]["fp64_queries"] < frac * out[2][1]["fp64_queries"]
+    if scene == "airframe":
+        ctx.destroy_mesh(mesh)
+
+
 def test_tri_duct_sigma_fp64_and_methods(ctx, tri_duct):
     v, f, mesh, s32, s64 = tri_duct
     kw = dict(C2S, theta_samples=4, phi_samples=4, theta_end=90, phi_end=90)
