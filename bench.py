@@ -302,7 +302,8 @@ def other_workload(args):
                                 "median": float(np.median(derr)),
                                 "max_within_30dB_of_peak": float(derr[strong].max()),
                                 "frac_angles_le_0.1dB": float(np.mean(derr <= 0.1))},
-        "retraced_rays": own["retraced"],
+        "undecided_rays": own["retraced"],
+        "full_fp64_queries": own["fp64_queries"],
         "counters_fp64_sample": cnt,
     }
 
@@ -413,7 +414,8 @@ def main():
                                 "max_within_30dB_of_peak": float(derr[strong].max()),
                                 "frac_angles_le_0.1dB": float(np.mean(derr <= 0.1))},
         "counters_fp64_traversal": cnt,
-        "retraced_rays": own["retraced"],
+        "undecided_rays": own["retraced"],
+        "full_fp64_queries": own["fp64_queries"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel": "trace_kernel", "kernel_ms": round(k_ms, 3), "algorithmic_bytes": int(alg_bytes)},

@@ -3,6 +3,7 @@
 #include "bvh_build.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -352,6 +353,90 @@ template <typename R> bool to_gpu_layout(const Bvh<R>& b, GpuBvh<R>& out, std::s
     }
     out.rootRef = ref_of(b.root);
     out.nInternal = ni;
+    return true;
+}
+
+bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
+    const int nn = static_cast<int>(b.nodes.size());
+    if (nn == 0) {
+        err = "empty mesh";
+        return false;
+    }
+    auto area = [&](int i) {
+        const RefNode<float>& n = b.nodes[i];
+        const double x = double(n.hi[0]) - n.lo[0], y = double(n.hi[1]) - n.lo[1], z = double(n.hi[2]) - n.lo[2];
+        return x * y + y * z + z * x;
+    };
+    auto leaf_ref = [&](int i) { return encode_leaf(b.nodes[i].firstTri, b.nodes[i].triCount); };
+    if (b.nodes[b.root].triCount != 0) {
+        out.nodes.assign(128, 0);
+        out.rootRef = leaf_ref(b.root);
+        out.nInternal = 0;
+        out.depth = 0;
+        return true;
+    }
+    std::vector<std::array<float, 32>> recs;
+    int maxDepth = 0;
+    // explicit work list: (binary node, record slot to fill, depth)
+    struct Item {
+        int node, rec, depth;
+    };
+    std::vector<Item> work;
+    recs.emplace_back();
+    work.push_back({b.root, 0, 1});
+    while (!work.empty()) {
+        const Item it = work.back();
+        work.pop_back();
+        maxDepth = std::max(maxDepth, it.depth);
+        // open internal members, largest surface area first, up to four
+        int kids[4] = {b.nodes[it.node].left, b.nodes[it.node].right, -1, -1};
+        int nk = 2;
+        while (nk < 4) {
+            int best = -1;
+            double bestA = -1.0;
+            for (int q = 0; q < nk; ++q)
+                if (b.nodes[kids[q]].triCount == 0 && area(kids[q]) > bestA) {
+                    bestA = area(kids[q]);
+                    best = q;
+                }
+            if (best < 0) break;
+            const int open = kids[best];
+            for (int q = nk; q > best + 1; --q) kids[q] = kids[q - 1];
+            kids[best] = b.nodes[open].left;
+            kids[best + 1] = b.nodes[open].right;
+            ++nk;
+        }
+        std::array<float, 32> r{};
+        int refs[4] = {kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot};
+        for (int q = 0; q < 4; ++q) {
+            if (q >= nk) {
+                for (int k = 0; k < 3; ++k) {
+                    r[4 * k + q] = 0.f;
+                    r[12 + 4 * k + q] = 0.f;
+                }
+                continue;
+            }
+            const RefNode<float>& c = b.nodes[kids[q]];
+            for (int k = 0; k < 3; ++k) {
+                r[4 * k + q] = c.lo[k];
+                r[12 + 4 * k + q] = c.hi[k];
+            }
+            if (c.triCount != 0) {
+                refs[q] = leaf_ref(kids[q]);
+            } else {
+                refs[q] = static_cast<int>(recs.size());
+                recs.emplace_back();
+                work.push_back({kids[q], refs[q], it.depth + 1});
+            }
+        }
+        std::memcpy(&r[24], refs, sizeof(refs));
+        recs[it.rec] = r;
+    }
+    out.nodes.resize(recs.size() * 128);
+    std::memcpy(out.nodes.data(), recs.data(), out.nodes.size());
+    out.rootRef = 0;
+    out.nInternal = static_cast<int>(recs.size());
+    out.depth = maxDepth;
     return true;
 }
 

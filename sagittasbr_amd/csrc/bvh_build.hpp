@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include "sbr_device.hpp"
+
 namespace sbr {
 
 struct BuildOptions {
@@ -56,6 +58,21 @@ template <typename R> struct GpuBvh {
 };
 
 template <typename R> bool to_gpu_layout(const Bvh<R>& b, GpuBvh<R>& out, std::string& err);
+
+// Four-wide FP32 layout for the mixed mode's certified queries, whose
+// outcome does not depend on the tree or its visiting order (only on which
+// triangles are tested). Made by collapsing the binary tree: each record
+// takes up to four descendants, opening the child with the largest surface
+// area first; leaves are the binary tree's leaves (same references).
+// Record 128 B: lo.x[4] lo.y[4] lo.z[4] hi.x[4] hi.y[4] hi.z[4] ref[4] pad[4];
+// an unused slot holds ref kEmptySlot.
+struct GpuBvh4 {
+    std::vector<uint8_t> nodes;
+    int rootRef = 0;
+    int nInternal = 0;
+    int depth = 0;  // records on the longest root-to-leaf path
+};
+bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err);
 
 inline int encode_leaf(int firstTri, int count) { return ~((firstTri << 4) | (count - 1)); }
 

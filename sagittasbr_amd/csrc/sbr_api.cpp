@@ -85,6 +85,8 @@ template <typename R> struct PrecMesh {
     GpuBvh<R> gpu;
     DevBuf nodes, tris;
     DevBuf tris64;  // FP32 tree: FP64 vertex records in tree order (80 B each)
+    GpuBvh4 gpu4;   // FP32 tree: four-wide collapse (mixed mode)
+    DevBuf nodes4;
     double buildMs = 0;
 };
 
@@ -179,6 +181,10 @@ template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
             long long fid = static_cast<long long>(f);
             std::memcpy(&t64[10 * h + 9], &fid, sizeof(double));
         }
+        if (!to_gpu_layout4(pm.bvh, pm.gpu4, err)) return fail(&ctx->err, err);
+        HIP_TRY(&ctx->err, pm.nodes4.ensure(pm.gpu4.nodes.size()));
+        HIP_TRY(&ctx->err,
+                hipMemcpy(pm.nodes4.p, pm.gpu4.nodes.data(), pm.gpu4.nodes.size(), hipMemcpyHostToDevice));
         HIP_TRY(&ctx->err, pm.tris64.ensure(t64.size() * sizeof(double)));
         HIP_TRY(&ctx->err, hipMemcpy(pm.tris64.p, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
     }
@@ -205,6 +211,10 @@ template <typename R> int upload_table(sbr_ctx* ctx) {
             mm.nodes = pm.nodes.p;
             mm.tris = pm.tris.p;
             mm.tris64 = pm.tris64.p;
+            if constexpr (std::is_same_v<R, float>) {
+                mm.nodes4 = pm.nodes4.p;
+                mm.rootRef4 = pm.gpu4.rootRef;
+            }
             for (int k = 0; k < 3; ++k) {
                 mm.rootLo[k] = pm.gpu.rootLo[k];
                 mm.rootHi[k] = pm.gpu.rootHi[k];
@@ -346,7 +356,12 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
             make_frame<R>(R(j.theta_deg), R(j.phi_deg), R(j.grid_size), J.llc, J.uv, J.vv, J.dir);
             make_phys<R>(j, J);
         } else {
-            // reference FP64 job for re-traces
+            // reference FP64 job: the ray state and the FP64 answers
+            if constexpr (std::is_same_v<R, float>) {
+                if (pm.gpu4.depth > kMaxDepth4)
+                    return fail(&ctx->err, "mixed mode: four-wide tree depth " + std::to_string(pm.gpu4.depth) +
+                                               " exceeds " + std::to_string(kMaxDepth4));
+            }
             if (ensure_built<double>(ctx, m)) return 1;
             Job<double> D{};
             make_frame<double>(j.theta_deg, j.phi_deg, j.grid_size, D.llc, D.uv, D.vv, D.dir);

@@ -10,6 +10,9 @@ constexpr int kMaxStack = 64;  // deferred-left capacity; equals the reference's
                                // int stack[64] (src/cuda/rtKernels.cu:65-66)
 constexpr int kMaxDepth = 62;  // deepest tree for which the reference never
                                // drops a push (stackPtr + 2 <= 64)
+constexpr int kMaxStack4 = 96; // mixed mode's four-wide tree: up to 3 pushes per record
+constexpr int kMaxDepth4 = 32; // deepest four-wide tree (3 * depth <= kMaxStack4)
+constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a four-wide record
 
 // One sigma value (mesh, f, theta, phi) ready for the device. Frame and
 // physics constants are in the working precision R; kd / twoPid carry the
@@ -38,8 +41,11 @@ template <typename R> struct alignas(16) Mesh {
     // FP32 tree's order (v0 v1 v2 as 9 doubles + face id), for the mixed
     // mode's FP64 recompute of a hit
     const void* tris64;
+    // FP32 tree only: the four-wide collapse of the tree (mixed mode)
+    const void* nodes4;
     R rootLo[3], rootHi[3];
     int rootRef;
+    int rootRef4;
     int nTris;
 };
 

@@ -195,6 +195,25 @@ __device__ __forceinline__ NodeRec<double> load_node(const double* nodes, int i)
     return n;
 }
 
+// Four-wide record of the mixed mode's tree: child boxes as x/y/z vectors of
+// four, then the four child references.
+struct Node4 {
+    vf4 lx, ly, lz, hx, hy, hz;
+    vi4 ref;
+};
+__device__ __forceinline__ Node4 load_node4(const void* nodes4, int i) {
+    gptr<vf4> p = as_global<vf4>(nodes4) + 8 * i;
+    Node4 n;
+    n.lx = p[0];
+    n.ly = p[1];
+    n.lz = p[2];
+    n.hx = p[3];
+    n.hy = p[4];
+    n.hz = p[5];
+    n.ref = as_global<vi4>(nodes4)[8 * i + 6];
+    return n;
+}
+
 // Moller-Trumbore, rtKernels.cu:10-44. Accepts t in [0.001, closest].
 template <typename R>
 __device__ __forceinline__ bool mt_test(const TriRec<R>& tr, V3<R> o, V3<R> d, R closest, R& tOut) {
@@ -413,7 +432,6 @@ template <int SD, bool CNT, bool WW>
 __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> d, V3<float> inv,
                                                  float& tHit, int* sRef, float* sE, Ov<float>::T* ov, int tid,
                                                  Work& w, float ex, float ed, float K, Cands& cand, bool& amb) {
-    const float* nodes = static_cast<const float*>(m.nodes);
     const float* tris = static_cast<const float*>(m.tris);
     const float o1 = fabsf(o.x) + fabsf(o.y) + fabsf(o.z);
     // far distance of anything in the root box from the origin
@@ -442,26 +460,46 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
         return -1;
     }
     LaneStack<float, SD> st{sRef, sE, ov, tid, 0};
-    int cur = m.rootRef;
-    // the true closest distance may lie up to tBestErr beyond the FP32 one;
-    // a box is pruned only when it starts beyond that as well
+    int cur = m.rootRef4;
+    // Four-wide tree: the hit children are visited nearest first (the
+    // outcome does not depend on the order). The true closest distance may
+    // lie up to tBestErr beyond the FP32 one; a box is pruned only when it
+    // starts beyond that as well.
     auto node_step = [&]() -> bool {
         const float lim = closest + tBestErr + tErr;
-        NodeRec<float> n = load_node(nodes, cur);
-        if (CNT) w.visits += 2;
-        float eL, eR;
-        bool hL = slab_grown(n.b[0], n.b[1], n.b[2], n.b[3], n.b[4], n.b[5], pLo, pHi, inv, eL) && eL <= lim;
-        bool hR = slab_grown(n.b[6], n.b[7], n.b[8], n.b[9], n.b[10], n.b[11], pLo, pHi, inv, eR) && eR <= lim;
-        if (hR) {
-            if (hL) st.push(eL, n.l);
-            cur = n.r;
-            return true;
+        const Node4 n = load_node4(m.nodes4, cur);
+        if (CNT) w.visits += 4;
+        float e[4];
+        int r[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float ent;
+            const bool h = slab_grown(n.lx[q], n.ly[q], n.lz[q], n.hx[q], n.hy[q], n.hz[q], pLo, pHi, inv, ent) &&
+                           ent <= lim && n.ref[q] != kEmptySlot;
+            e[q] = h ? ent : 3.0e38f;
+            r[q] = n.ref[q];
         }
-        if (hL) {
-            cur = n.l;
-            return true;
-        }
-        return false;
+        auto cswap = [&](int a, int b) {
+            if (e[b] < e[a]) {
+                const float te = e[a];
+                e[a] = e[b];
+                e[b] = te;
+                const int tr = r[a];
+                r[a] = r[b];
+                r[b] = tr;
+            }
+        };
+        cswap(0, 1);
+        cswap(2, 3);
+        cswap(0, 2);
+        cswap(1, 3);
+        cswap(1, 2);
+        if (!(e[0] < 1.0e38f)) return false;
+        if (e[3] < 1.0e38f) st.push(e[3], r[3]);
+        if (e[2] < 1.0e38f) st.push(e[2], r[2]);
+        if (e[1] < 1.0e38f) st.push(e[1], r[1]);
+        cur = r[0];
+        return true;
     };
     auto leaf_tests = [&]() {
         int code = ~cur;
@@ -903,10 +941,12 @@ constexpr int kPoolRays = 1024;  // rays per pool batch (4 passes of 16x16)
 // mode (1 = no constraint); overridable at build time. Measured on the
 // trihedral+duct sweep and a 10 GHz airframe sweep (LDS stack 8): the
 // unconstrained mixed kernel takes ~150 VGPRs (3 waves) and runs 12-16%
-// slower than when held to 5 waves with a few spills; FP64 gains ~10% at
-// 4 waves, FP32 ~2-8% at 8. Deeper LDS stacks are bound by LDS instead.
+// slower; held to 4 waves (128 VGPRs) it beats 5 waves by 4% on the
+// trihedral+duct sweep and ties on the airframe, and 6 waves (80 VGPRs,
+// spills in the traversal loop) loses 8%. FP64 gains ~10% at 4 waves, FP32
+// ~2-8% at 8. Deeper LDS stacks are bound by LDS instead.
 #ifndef SBR_WPE_MIXED
-#define SBR_WPE_MIXED 5
+#define SBR_WPE_MIXED 4
 #endif
 #ifndef SBR_WPE_FP64
 #define SBR_WPE_FP64 4
@@ -938,7 +978,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     const int qx = (wave & 1) * 8 + (lane & 7);
     const int qy = (wave >> 1) * 8 + (lane >> 3);
     unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0, cRetrace = 0, cFull = 0;
-    typename Ov<R>::T ov[kMaxStack - SD];
+    typename Ov<R>::T ov[(MODE == 1 ? kMaxStack4 : kMaxStack) - SD];
     // mixed mode: private stack of the FP64 queries that answer ambiguous ones
     Ov<double>::T ov64[MODE == 1 ? kMaxStack : 1];
 
