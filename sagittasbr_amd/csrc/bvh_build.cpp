@@ -357,6 +357,8 @@ template <typename R> bool to_gpu_layout(const Bvh<R>& b, GpuBvh<R>& out, std::s
 }
 
 bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
+    constexpr int W = kWide;
+    constexpr int F = kWideRecBytes / 4;  // floats per record
     const int nn = static_cast<int>(b.nodes.size());
     if (nn == 0) {
         err = "empty mesh";
@@ -369,13 +371,13 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
     };
     auto leaf_ref = [&](int i) { return encode_leaf(b.nodes[i].firstTri, b.nodes[i].triCount); };
     if (b.nodes[b.root].triCount != 0) {
-        out.nodes.assign(128, 0);
+        out.nodes.assign(kWideRecBytes, 0);
         out.rootRef = leaf_ref(b.root);
         out.nInternal = 0;
         out.depth = 0;
         return true;
     }
-    std::vector<std::array<float, 32>> recs;
+    std::vector<std::array<float, F>> recs;
     int maxDepth = 0;
     // explicit work list: (binary node, record slot to fill, depth)
     struct Item {
@@ -388,10 +390,12 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
         const Item it = work.back();
         work.pop_back();
         maxDepth = std::max(maxDepth, it.depth);
-        // open internal members, largest surface area first, up to four
-        int kids[4] = {b.nodes[it.node].left, b.nodes[it.node].right, -1, -1};
+        // open internal members, largest surface area first, up to W
+        int kids[W];
+        kids[0] = b.nodes[it.node].left;
+        kids[1] = b.nodes[it.node].right;
         int nk = 2;
-        while (nk < 4) {
+        while (nk < W) {
             int best = -1;
             double bestA = -1.0;
             for (int q = 0; q < nk; ++q)
@@ -406,20 +410,15 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
             kids[best + 1] = b.nodes[open].right;
             ++nk;
         }
-        std::array<float, 32> r{};
-        int refs[4] = {kEmptySlot, kEmptySlot, kEmptySlot, kEmptySlot};
-        for (int q = 0; q < 4; ++q) {
-            if (q >= nk) {
-                for (int k = 0; k < 3; ++k) {
-                    r[4 * k + q] = 0.f;
-                    r[12 + 4 * k + q] = 0.f;
-                }
-                continue;
-            }
+        std::array<float, F> r{};
+        int refs[W];
+        for (int q = 0; q < W; ++q) {
+            refs[q] = kEmptySlot;
+            if (q >= nk) continue;
             const RefNode<float>& c = b.nodes[kids[q]];
             for (int k = 0; k < 3; ++k) {
-                r[4 * k + q] = c.lo[k];
-                r[12 + 4 * k + q] = c.hi[k];
+                r[W * k + q] = c.lo[k];
+                r[W * (3 + k) + q] = c.hi[k];
             }
             if (c.triCount != 0) {
                 refs[q] = leaf_ref(kids[q]);
@@ -429,10 +428,10 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
                 work.push_back({kids[q], refs[q], it.depth + 1});
             }
         }
-        std::memcpy(&r[24], refs, sizeof(refs));
+        std::memcpy(&r[6 * W], refs, sizeof(refs));
         recs[it.rec] = r;
     }
-    out.nodes.resize(recs.size() * 128);
+    out.nodes.resize(recs.size() * kWideRecBytes);
     std::memcpy(out.nodes.data(), recs.data(), out.nodes.size());
     out.rootRef = 0;
     out.nInternal = static_cast<int>(recs.size());

@@ -59,12 +59,12 @@ template <typename R> struct GpuBvh {
 
 template <typename R> bool to_gpu_layout(const Bvh<R>& b, GpuBvh<R>& out, std::string& err);
 
-// Four-wide FP32 layout for the mixed mode's certified queries, whose
-// outcome does not depend on the tree or its visiting order (only on which
-// triangles are tested). Made by collapsing the binary tree: each record
-// takes up to four descendants, opening the child with the largest surface
-// area first; leaves are the binary tree's leaves (same references).
-// Record 128 B: lo.x[4] lo.y[4] lo.z[4] hi.x[4] hi.y[4] hi.z[4] ref[4] pad[4];
+// Wide FP32 layout for the mixed mode's certified queries, whose outcome
+// does not depend on the tree or its visiting order (only on which triangles
+// are tested). Made by collapsing the binary tree: each record takes up to
+// kWide descendants, opening the child with the largest surface area first;
+// leaves are the binary tree's leaves (same references). Record of
+// kWideRecBytes: lo.x[W] lo.y[W] lo.z[W] hi.x[W] hi.y[W] hi.z[W] ref[W], pad;
 // an unused slot holds ref kEmptySlot.
 struct GpuBvh4 {
     std::vector<uint8_t> nodes;

@@ -10,9 +10,13 @@ constexpr int kMaxStack = 64;  // deferred-left capacity; equals the reference's
                                // int stack[64] (src/cuda/rtKernels.cu:65-66)
 constexpr int kMaxDepth = 62;  // deepest tree for which the reference never
                                // drops a push (stackPtr + 2 <= 64)
-constexpr int kMaxStack4 = 96; // mixed mode's four-wide tree: up to 3 pushes per record
-constexpr int kMaxDepth4 = 32; // deepest four-wide tree (3 * depth <= kMaxStack4)
-constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a four-wide record
+// Mixed mode's wide tree: kWide children per record, up to kWide - 1 pushes
+// per record visited.
+constexpr int kWide = 4;
+constexpr int kWideRecBytes = (kWide * 28 + 63) / 64 * 64;  // 6 box floats + 1 ref per child, padded
+constexpr int kMaxStack4 = 96;
+constexpr int kMaxDepth4 = kMaxStack4 / (kWide - 1);
+constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a wide record
 
 // One sigma value (mesh, f, theta, phi) ready for the device. Frame and
 // physics constants are in the working precision R; kd / twoPid carry the

@@ -195,23 +195,49 @@ __device__ __forceinline__ NodeRec<double> load_node(const double* nodes, int i)
     return n;
 }
 
-// Four-wide record of the mixed mode's tree: child boxes as x/y/z vectors of
-// four, then the four child references.
-struct Node4 {
-    vf4 lx, ly, lz, hx, hy, hz;
-    vi4 ref;
+// Wide record of the mixed mode's tree: lo.x[W] lo.y[W] lo.z[W] hi.x[W]
+// hi.y[W] hi.z[W] ref[W] as 16-byte loads.
+struct NodeW {
+    float f[7 * kWide];
+    __device__ __forceinline__ int ref(int q) const { return __float_as_int(f[6 * kWide + q]); }
 };
-__device__ __forceinline__ Node4 load_node4(const void* nodes4, int i) {
-    gptr<vf4> p = as_global<vf4>(nodes4) + 8 * i;
-    Node4 n;
-    n.lx = p[0];
-    n.ly = p[1];
-    n.lz = p[2];
-    n.hx = p[3];
-    n.hy = p[4];
-    n.hz = p[5];
-    n.ref = as_global<vi4>(nodes4)[8 * i + 6];
+__device__ __forceinline__ NodeW load_nodew(const void* nodes, int i) {
+    gptr<vf4> p = as_global<vf4>(nodes) + (kWideRecBytes / 16) * i;
+    NodeW n;
+#pragma unroll
+    for (int k = 0; k < 7 * kWide / 4; ++k) {
+        const vf4 v = p[k];
+        n.f[4 * k] = v.x;
+        n.f[4 * k + 1] = v.y;
+        n.f[4 * k + 2] = v.z;
+        n.f[4 * k + 3] = v.w;
+    }
     return n;
+}
+
+// Sorting network on (key, value) pairs, ascending keys.
+template <int W> __device__ __forceinline__ void sort_net(float* e, int* r) {
+    auto cs = [&](int a, int b) {
+        if (e[b] < e[a]) {
+            const float te = e[a];
+            e[a] = e[b];
+            e[b] = te;
+            const int tr = r[a];
+            r[a] = r[b];
+            r[b] = tr;
+        }
+    };
+    if constexpr (W == 4) {
+        cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+    } else {
+        static_assert(W == 8, "wide records of 4 or 8 children");
+        cs(0, 1); cs(2, 3); cs(4, 5); cs(6, 7);
+        cs(0, 2); cs(1, 3); cs(4, 6); cs(5, 7);
+        cs(1, 2); cs(5, 6);
+        cs(0, 4); cs(1, 5); cs(2, 6); cs(3, 7);
+        cs(2, 4); cs(3, 5);
+        cs(1, 2); cs(3, 4); cs(5, 6);
+    }
 }
 
 // Moller-Trumbore, rtKernels.cu:10-44. Accepts t in [0.001, closest].
@@ -461,43 +487,31 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     }
     LaneStack<float, SD> st{sRef, sE, ov, tid, 0};
     int cur = m.rootRef4;
-    // Four-wide tree: the hit children are visited nearest first (the
-    // outcome does not depend on the order). The true closest distance may
-    // lie up to tBestErr beyond the FP32 one; a box is pruned only when it
-    // starts beyond that as well.
+    // Wide tree: the hit children are visited nearest first (the outcome
+    // does not depend on the order). The true closest distance may lie up to
+    // tBestErr beyond the FP32 one; a box is pruned only when it starts
+    // beyond that as well.
     auto node_step = [&]() -> bool {
+        constexpr int W = kWide;
         const float lim = closest + tBestErr + tErr;
-        const Node4 n = load_node4(m.nodes4, cur);
-        if (CNT) w.visits += 4;
-        float e[4];
-        int r[4];
+        const NodeW n = load_nodew(m.nodes4, cur);
+        if (CNT) w.visits += W;
+        float e[W];
+        int r[W];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < W; ++q) {
             float ent;
-            const bool h = slab_grown(n.lx[q], n.ly[q], n.lz[q], n.hx[q], n.hy[q], n.hz[q], pLo, pHi, inv, ent) &&
-                           ent <= lim && n.ref[q] != kEmptySlot;
+            const bool h = slab_grown(n.f[q], n.f[W + q], n.f[2 * W + q], n.f[3 * W + q], n.f[4 * W + q],
+                                      n.f[5 * W + q], pLo, pHi, inv, ent) &&
+                           ent <= lim && n.ref(q) != kEmptySlot;
             e[q] = h ? ent : 3.0e38f;
-            r[q] = n.ref[q];
+            r[q] = n.ref(q);
         }
-        auto cswap = [&](int a, int b) {
-            if (e[b] < e[a]) {
-                const float te = e[a];
-                e[a] = e[b];
-                e[b] = te;
-                const int tr = r[a];
-                r[a] = r[b];
-                r[b] = tr;
-            }
-        };
-        cswap(0, 1);
-        cswap(2, 3);
-        cswap(0, 2);
-        cswap(1, 3);
-        cswap(1, 2);
+        sort_net<W>(e, r);
         if (!(e[0] < 1.0e38f)) return false;
-        if (e[3] < 1.0e38f) st.push(e[3], r[3]);
-        if (e[2] < 1.0e38f) st.push(e[2], r[2]);
-        if (e[1] < 1.0e38f) st.push(e[1], r[1]);
+#pragma unroll
+        for (int q = W - 1; q >= 1; --q)
+            if (e[q] < 1.0e38f) st.push(e[q], r[q]);
         cur = r[0];
         return true;
     };
