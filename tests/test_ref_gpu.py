@@ -114,12 +114,27 @@ def test_per_ray_vs_reference_build(ctx, case, prec):
 
 def test_reference_timing(ctx):
     """Reference per-angle kernels vs libsbr on the sphere phi cut (360
-    angles, 250x250 rays) and every 4th angle of the trihedral+duct sweep
-    (535x535 rays, 10 bounces)."""
+    angles, 250x250 rays), every 4th angle of the trihedral+duct sweep
+    (535x535 rays, 10 bounces) and 12 evenly spread angles of each frequency
+    of the airframe sweep (3.2M triangles, 733^2 / 2200^2 / 5864^2 rays,
+    8 bounces).
+
+    Same-precision agreement is asserted where FP32 rounding stays below the
+    sums' sensitivity (sphere, trihedral+duct). On the airframe the FP32
+    phase 2 k L (k up to 209 rad/m, paths of ~30 m) carries ~1e-3 rad of
+    rounding, which differs between the reference build (FP32 contracted
+    into FMAs by its compiler flags) and libsbr's FP32 path (FMAs at the
+    reference source's fixed places), so the two FP32 sums differ by a few
+    hundredths of a dB at the median; the mixed mode has the FP64 phase."""
+    from sagittasbr_amd import workloads as W
+
     _need("f32")
     out = {}
+    c3 = W.c3()
     cases = [("sphere_c1", M.icosphere(5), dict(C1S, phi_end=359, phi_samples=360), 1),
              ("tri_duct_c2", M.trihedral_duct_scene(), dict(C2S, n_x=535, n_y=535), 4)]
+    for (mi, kw3), fname in zip(c3["sweeps"], ("1ghz", "3ghz", "10ghz")):
+        cases.append((f"airframe_c3_{fname}", c3["meshes"][mi], kw3, 930))
     for name, (v, f), kw, step in cases:
         p, op = S.SweepParams.make(**kw), O.Params(**kw)
         angles = np.arange(0, p.n_angles, step, dtype=np.int32)
@@ -166,5 +181,17 @@ def test_reference_timing(ctx):
         # swing the deepest nulls of the near-incoherent multi-bounce sweep by
         # up to dB; the typical angle agrees to ~1e-5 dB. (The mixed mode has
         # the FP64 outcome, which the FP32 build drifts from by ~0.5 dB here.)
-        assert res["per_angle_fp32_db_vs_reference"]["median"] <= 1e-3
+        if not name.startswith("airframe"):
+            assert res["per_angle_fp32_db_vs_reference"]["median"] <= 1e-3
+        else:
+            assert res["per_angle_fp32_db_vs_reference"]["median"] <= 0.5
+    # the whole airframe sweep (11,160 angles per frequency) at the sampled
+    # per-angle costs, against the batched mixed sweep's own sampled cost
+    air = [out[k] for k in out if k.startswith("airframe")]
+    if air:
+        ref_s = sum(11160 * r["reference_fp32_kernels_ms"] / r["angles"] for r in air) / 1000
+        mix_s = sum(11160 * r["batched_mixed_ms"] / r["angles"] for r in air) / 1000
+        out["airframe_c3_estimate"] = {"reference_fp32_kernels_s": round(ref_s, 2), "batched_mixed_s": round(mix_s, 2),
+                                       "batched_mixed_over_reference": round(ref_s / mix_s, 2)}
+        print("   ", json.dumps(out["airframe_c3_estimate"]), flush=True)
     print("REF_GPU_TIMING " + json.dumps(out), flush=True)
