@@ -91,9 +91,6 @@ typedef struct {
     int traversal;   /* loop shape of the BVH traversal: 1 = if-if (a wavefront runs node
                         steps and leaf tests side by side), 2 = while-while (leaves are
                         tested once every lane has reached one; default) */
-    int mixed_fp64;  /* SBR_MIXED, queries FP32 cannot decide: 1 = the reference FP64 test of
-                        their candidate triangles where that decides, else the full FP64
-                        query (default); 2 = always the full FP64 query on the FP64 tree */
 } sbr_tuning;
 
 /* Mesh / BVH statistics. */
@@ -113,10 +110,9 @@ typedef struct {
     uint64_t tri_tests;   /* Moller-Trumbore tests */
     uint64_t ray_bounces; /* sum over rays of hitCount */
     uint64_t queries;     /* closest-hit queries */
-    uint64_t retraced;    /* mixed mode: rays with an undecided query (an FP32 decision fell
-                             inside its error bound), answered in FP64 */
-    uint64_t fp64_queries; /* mixed mode: undecided queries answered by a full FP64 query on the
-                              FP64 tree (the others by FP64 tests of their candidates) */
+    uint64_t retraced;    /* mixed mode: rays with an undecided query (a near-tie, or a hit on
+                             the edge of its box), answered by the full FP64 query */
+    uint64_t fp64_queries; /* mixed mode: undecided queries */
 } sbr_counters;
 
 const char* sbr_last_error(const sbr_ctx* ctx);

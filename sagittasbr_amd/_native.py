@@ -93,7 +93,7 @@ class Job(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("wg_per_cu", C.c_int), ("lds_stack", C.c_int), ("tile_rays", C.c_int), ("use_graph", C.c_int),
-                ("ray_pool", C.c_int), ("traversal", C.c_int), ("mixed_fp64", C.c_int)]
+                ("ray_pool", C.c_int), ("traversal", C.c_int)]
 
 
 class MeshInfo(C.Structure):
@@ -287,8 +287,8 @@ class Plan:
         self.h = _P()
         _check(lib().sbr_plan_create(ctx.h, C.cast(jobs, _P), self.n, precision, method, C.byref(self.h)), ctx.h)
 
-    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0, ray_pool=0, traversal=0, mixed_fp64=0):
-        t = Tuning(wg_per_cu, lds_stack, tile_rays, use_graph, ray_pool, traversal, mixed_fp64)
+    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0, ray_pool=0, traversal=0):
+        t = Tuning(wg_per_cu, lds_stack, tile_rays, use_graph, ray_pool, traversal)
         _check(lib().sbr_plan_set_tuning(self.h, C.byref(t)), self.ctx.h)
 
     def autotune(self, sample_jobs=0, repeats=1):

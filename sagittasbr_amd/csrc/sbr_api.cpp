@@ -509,7 +509,6 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
         a.jobs64 = static_cast<const Job<double>*>(pl->dJobs64.p);
         a.meshes64 = static_cast<const Mesh<double>*>(ctx->meshTable64.p);
         a.errK = kErrSafety;
-        a.localFp64 = pl->tune.mixed_fp64 == 2 ? 0 : 1;
         a.recFlag = static_cast<int8_t*>(pl->dRecFlag.p);
     }
     hipStream_t s = ctx->stream;
@@ -948,7 +947,6 @@ int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
     if (t->lds_stack < 0 || t->wg_per_cu < 0) return fail(&pl->ctx->err, "negative tuning value");
     if (t->ray_pool < 0 || t->ray_pool > 2) return fail(&pl->ctx->err, "ray_pool must be 0, 1 or 2");
     if (t->traversal < 0 || t->traversal > 2) return fail(&pl->ctx->err, "traversal must be 0, 1 or 2");
-    if (t->mixed_fp64 < 0 || t->mixed_fp64 > 2) return fail(&pl->ctx->err, "mixed_fp64 must be 0, 1 or 2");
     if (t->tile_rays != 0 && t->tile_rays != 16 && t->tile_rays != 32 && t->tile_rays != 64)
         return fail(&pl->ctx->err, "tile_rays must be 16, 32 or 64");
     drop_graph(pl);
@@ -1019,9 +1017,9 @@ int sbr_plan_count(sbr_plan* pl, sbr_counters* out) {
     unsigned long long c[16] = {0};
     HIP_TRY(&ctx->err, hipMemcpyAsync(c, pl->dCounters.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
-    // rays, visits, tests, bounces, queries (each query counted once, full
-    // FP64 answers of undecided queries included), rays with an undecided
-    // query, undecided queries answered by a full FP64 query
+    // rays, visits, tests, bounces, queries (each query counted once, FP64
+    // answers of undecided queries included), rays with an undecided query,
+    // undecided queries
     out->rays = c[0];
     out->node_visits = c[1];
     out->tri_tests = c[2];
@@ -1060,8 +1058,6 @@ int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* be
                     for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool, 0});
             }
         }
-    // the answer mode of undecided mixed queries is the caller's choice
-    for (sbr_tuning& c : cands) c.mixed_fp64 = pl->tune.mixed_fp64;
     double bestT = 1e300;
     sbr_tuning bestC = cands[0];
     int rc = 0, tried = 0;

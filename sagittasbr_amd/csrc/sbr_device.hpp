@@ -71,16 +71,14 @@ template <typename R> struct TraceArgs {
     double* recL;
     double* recReIm;
     // work counters: rays, visits, tests, bounces, queries, rays with an
-    // undecided query, undecided queries answered by a full FP64 query
+    // undecided query, undecided queries (answered by the full FP64 query)
     unsigned long long* counters;
     // mixed mode: reference FP64 jobs and meshes (FP64 ray state and the
-    // queries that FP32 cannot decide), error-bound safety factor, how
-    // undecided queries are answered, per-ray flags of rays with such a
-    // query (records only)
+    // queries the mixed query cannot vouch for), error-bound safety factor,
+    // per-ray flags of rays with such a query (records only)
     const Job<double>* jobs64;
     const Mesh<double>* meshes64;
     float errK;
-    int localFp64;  // 1: undecided queries answered from their candidates where these decide
     int8_t* recFlag;
 };
 

@@ -409,56 +409,29 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
 
 // ------------------------------------------------- mixed-mode traversal
 constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
-constexpr int kCand = 4;                 // candidate slots per lane
 
-// Candidate triangles of an undecided mixed query, kept per lane in LDS at
-// [slot * kWgSize + tid]: every certified hit that may tie the best one and
-// every triangle whose FP64 outcome is open (with the lowest distance it can
-// have), as long as it may beat or tie the best certified hit.
-struct Cands {
-    int* ids;
-    float* lbs;
-    int tid;
-    int n;
-    bool over;  // a relevant candidate found no slot
-    // drop the entries beyond lim, then append (idx, lb)
-    __device__ __forceinline__ void add(int idx, float lb, float lim) {
-        if (n == kCand) compact(lim);
-        if (n < kCand) {
-            ids[n * kWgSize + tid] = idx;
-            lbs[n * kWgSize + tid] = lb;
-            ++n;
-        } else {
-            over = true;
-        }
-    }
-    __device__ __forceinline__ void compact(float lim) {
-        int m = 0;
-        for (int q = 0; q < n; ++q) {
-            const float l = lbs[q * kWgSize + tid];
-            if (l <= lim) {
-                if (m != q) {
-                    ids[m * kWgSize + tid] = ids[q * kWgSize + tid];
-                    lbs[m * kWgSize + tid] = l;
-                }
-                ++m;
-            }
-        }
-        n = m;
-    }
-};
-
-// FP32 closest-hit query with error-bounded decisions. Returns the FP32-tree
-// position of the best certified hit (an FP64 hit too, at tHit +- its bound)
-// or -1; amb is set when some candidate may beat or tie it (the query is then
-// undecided in FP32 and the candidates are left in cand). Pruning uses the
-// best certified hit only, so every triangle that could be the FP64 answer
-// is tested.
+// Mixed-mode query: FP32 traversal of the wide tree, every box grown by the
+// position error bound of the FP32 copy of the ray (so no box the FP64 ray
+// reaches is missed), and the reference FP64 Moller-Trumbore test
+// (rtKernels.cu:10-44) of every triangle reached, on the FP64 ray and the
+// reference FP64 vertices. Returns the FP32-tree position of the nearest FP64
+// hit (or -1) and its FP64 distance. The reference query returns that
+// triangle too unless (a) another hit lies within the tie margin (the
+// reference breaks exact ties by its own tree's test order) or (b) the hit
+// point lies within the margin of the edge of the triangle's box on an axis
+// the triangle spans (the reference's FP64 box tests could reject it by
+// rounding; on an axis the triangle does not span, both slab planes of a
+// leaf give the same value); amb is set then. Rounding of the reference's
+// FP64 slab and distance arithmetic stays below 2^-51 (R + |o| + |P|) in
+// position units (R: the mesh's largest coordinate magnitude per axis,
+// summed), so the margin is 1e-12 of that. The FP32 origin may be moved
+// along the ray by shift (first query): FP32 box distances are then shift
+// shorter than the FP64 ones.
 template <int SD, bool CNT, bool WW>
-__device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> d, V3<float> inv,
-                                                 float& tHit, int* sRef, float* sE, Ov<float>::T* ov, int tid,
-                                                 Work& w, float ex, float ed, float K, Cands& cand, bool& amb) {
-    const float* tris = static_cast<const float*>(m.tris);
+__device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> inv, V3<double> o64,
+                                                 V3<double> d64, double shift, double& tHit, int* sRef, float* sE,
+                                                 Ov<float>::T* ov, int tid, Work& w, float ex, float ed, float K,
+                                                 bool& amb) {
     const float o1 = fabsf(o.x) + fabsf(o.y) + fabsf(o.z);
     // far distance of anything in the root box from the origin
     float cx = 0.5f * (m.rootLo[0] + m.rootHi[0]), cy = 0.5f * (m.rootLo[1] + m.rootHi[1]),
@@ -466,13 +439,12 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     float hx = m.rootHi[0] - cx, hy = m.rootHi[1] - cy, hz = m.rootHi[2] - cz;
     float tFar = sqrtf((o.x - cx) * (o.x - cx) + (o.y - cy) * (o.y - cy) + (o.z - cz) * (o.z - cz)) +
                  sqrtf(hx * hx + hy * hy + hz * hz);
-    // position and distance error bounds valid over the whole query
+    // position and distance error bounds of the FP32 box tests over the query
     float dx = ex + ed * tFar + K * kEps32 * (o1 + tFar);
     float tErr = dx + 4.0f * K * kEps32 * tFar;
-    float closest = 1e20f, tBestErr = 0.0f;
+    double closest = 1e20, second = 1e20;
     int hit = -1;
-    // lowest distance a rival may have and still beat or tie the best
-    auto rival_lim = [&]() { return closest + 2.0f * tBestErr + 1.0e-6f * closest; };
+    amb = false;
     float e;
     if (CNT) w.visits += 1;
     // o + dx and o - dx rounded outwards: the margin covers the rounding of
@@ -485,15 +457,17 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
         tHit = closest;
         return -1;
     }
+    // pruning bound in FP32 box distances: a box is dropped only when it
+    // starts beyond the best FP64 hit by more than the box test's error (and
+    // the tie margin, far below it)
+    float lim = 3.0e38f;
+    auto update_lim = [&]() { lim = float(closest - shift) * (1.0f + 4.0f * kEps32) + 2.0f * tErr; };
     LaneStack<float, SD> st{sRef, sE, ov, tid, 0};
     int cur = m.rootRef4;
-    // Wide tree: the hit children are visited nearest first (the outcome
-    // does not depend on the order). The true closest distance may lie up to
-    // tBestErr beyond the FP32 one; a box is pruned only when it starts
-    // beyond that as well.
+    // wide tree: the hit children are visited nearest first (the outcome
+    // does not depend on the order)
     auto node_step = [&]() -> bool {
         constexpr int W = kWide;
-        const float lim = closest + tBestErr + tErr;
         const NodeW n = load_nodew(m.nodes4, cur);
         if (CNT) w.visits += W;
         float e[W];
@@ -521,54 +495,30 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
         int cnt = (code & 15) + 1;
         if (CNT) w.tests += cnt;
         for (int k = 0; k < cnt; ++k) {
-            TriRec<float> tr = load_tri(tris, first + k);
-            V3<float> p = cross3(d, tr.e2);
-            float det = dot3(tr.e1, p);
-            float adet = fabsf(det);
-            float invd = 1.0f / det;
-            V3<float> tv = sub(o, tr.v0);
-            float u = dot3(tv, p) * invd;
-            V3<float> q = cross3(tv, tr.e1);
-            float v = dot3(d, q) * invd;
-            float t = dot3(tr.e2, q) * invd;
-            // forward error bounds of this test against the FP64 ray
-            float at = fabsf(t);
-            float v01 = fabsf(tr.v0.x) + fabsf(tr.v0.y) + fabsf(tr.v0.z);
-            float xerr = ex + ed * at + K * kEps32 * (o1 + v01 + 1.0f);
-            // >= 1 / |det|: invd is correctly rounded
-            float rdet = fminf(fabsf(invd), 1e30f) * (1.0f + 4.0f * kEps32);
-            float berr = xerr * tr.sumE * rdet + 4.0f * K * kEps32;
-            float terr = xerr * tr.prodE * rdet + K * kEps32 * (o1 + v01 + at);
-            float ddet = (ed + K * kEps32) * tr.prodE * 2.0f;
-            float mrg = fminf(u, fminf(v, 1.0f - u - v));
-            bool detOk = adet >= 1.0e-6f;
-            bool inside = detOk && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || u + v > 1.0f);
-            bool nearEdge = detOk && fabsf(mrg) <= berr;
-            bool ambig = (fabsf(adet - 1.0e-6f) <= ddet) || nearEdge ||
-                         ((inside || nearEdge) && fabsf(t - 0.001f) <= terr);
-            // certified: an FP64 hit at t +- terr; open: may or may not be
-            // an FP64 hit (at t +- terr if it is)
-            const bool cert = inside && !ambig && t >= 0.001f;
-            const bool open = ambig && t >= 0.001f - terr;
-            if (cert && t <= closest) {
-                // the replaced best stays a rival while it may tie the new one
-                if (hit >= 0 && closest - tBestErr <= t + 2.0f * terr + 1.0e-6f * t)
-                    cand.add(hit, closest - tBestErr, t + 2.0f * terr + 1.0e-6f * t);
-                closest = t;
-                tBestErr = terr;
-                hit = first + k;
-            } else if ((cert || open) && t - terr <= rival_lim()) {
-                cand.add(first + k, t - terr, rival_lim());
+            V3<double> a, b, c;
+            int f;
+            load_tri_raw64(m.tris64, first + k, a, b, c, f);
+            const TriRec<double> tr{a, sub(b, a), sub(c, a), f, 0.f, 0.f};
+            double t;
+            if (mt_test(tr, o64, d64, 1e20, t)) {
+                if (t < closest) {
+                    second = closest;
+                    closest = t;
+                    hit = first + k;
+                    update_lim();
+                } else if (t < second) {
+                    second = t;
+                }
             }
         }
     };
     if constexpr (WW) {
         for (;;) {
             while (cur >= 0)
-                if (!node_step()) cur = st.pop(closest + tBestErr + tErr);
+                if (!node_step()) cur = st.pop(lim);
             if (cur == kNoRef) break;
             leaf_tests();
-            cur = st.pop(closest + tBestErr + tErr);
+            cur = st.pop(lim);
             if (cur == kNoRef) break;
         }
     } else {
@@ -579,13 +529,31 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
             } else {
                 leaf_tests();
             }
-            cur = st.pop(closest + tBestErr + tErr);
+            cur = st.pop(lim);
             if (cur == kNoRef) break;
         }
     }
-    // the final best decides which candidates still matter
-    cand.compact(rival_lim());
-    amb = cand.over || cand.n > 0;
+    if (hit >= 0) {
+        const double R = 1.0 + fmax(fabs(double(m.rootLo[0])), fabs(double(m.rootHi[0]))) +
+                         fmax(fabs(double(m.rootLo[1])), fabs(double(m.rootHi[1]))) +
+                         fmax(fabs(double(m.rootLo[2])), fabs(double(m.rootHi[2])));
+        const V3<double> P = axpy(closest, d64, o64);
+        const double margin =
+            1e-12 * (R + fabs(o64.x) + fabs(o64.y) + fabs(o64.z) + fabs(P.x) + fabs(P.y) + fabs(P.z));
+        if (second - closest <= margin) {
+            amb = true;
+        } else {
+            V3<double> a, b, c;
+            int f;
+            load_tri_raw64(m.tris64, hit, a, b, c, f);
+            auto inside_axis = [&](double pa, double x0, double x1, double x2) {
+                const double lo = fmin(x0, fmin(x1, x2)), hi = fmax(x0, fmax(x1, x2));
+                return !(hi > lo) || (pa - lo > margin && hi - pa > margin);
+            };
+            amb = !inside_axis(P.x, a.x, b.x, c.x) || !inside_axis(P.y, a.y, b.y, c.y) ||
+                  !inside_axis(P.z, a.z, b.z, c.z);
+        }
+    }
     tHit = closest;
     return hit;
 }
@@ -784,75 +752,20 @@ __device__ __forceinline__ void ray_init_mixed(const Job<double>& D, int i, int 
     st.k = 0;
 }
 
-// FP64 answer of an undecided query from its candidates: the reference FP64
-// Moller-Trumbore test (rtKernels.cu:10-44) of each candidate and of the
-// certified best h. The reference query returns the nearest accepted
-// triangle, so that is the answer when (a) no other accepted candidate is
-// within 1e-12 of its distance (the reference breaks exact ties by test
-// order, which the FP32 tree does not share) and (b) its hit point lies in
-// the triangle's box with a margin far above FP64 rounding on every axis the
-// triangle spans, so the reference's box tests on the way to it cannot
-// reject it by rounding (on an axis the triangle does not span, both slab
-// planes of a leaf box give the same value). Every triangle that could be
-// the answer is among the candidates (pruning used certified hits only).
-// Rounding of the reference's FP64 slab and distance arithmetic stays below
-// 2^-51 (|o| + |P| + R) in position units (R: the mesh's largest coordinate
-// magnitude per axis, summed), so ties and margins are judged against 1e-12
-// of that. Returns the FP32-tree position of the answer, -1 for a miss, or
-// -3 (near tie), -4 (margin), -5 (the certified hit rejected) when the full
-// FP64 query must decide.
-__device__ __forceinline__ int resolve_fp64(const void* tris64, V3<double> o, V3<double> d, int h, const Cands& c,
-                                            double scale) {
-    double best = 3.0e300, second = 3.0e300;
-    int win = -1;
-    auto test = [&](int idx) {
-        V3<double> a, b, e;
-        int f;
-        load_tri_raw64(tris64, idx, a, b, e, f);
-        const TriRec<double> tr{a, sub(b, a), sub(e, a), f, 0.f, 0.f};
-        double t;
-        if (mt_test(tr, o, d, 1e20, t)) {
-            if (t < best) {
-                second = best;
-                best = t;
-                win = idx;
-            } else if (t < second) {
-                second = t;
-            }
-        }
-    };
-    if (h >= 0) test(h);
-    for (int q = 0; q < c.n; ++q) test(c.ids[q * kWgSize + c.tid]);
-    if (win < 0) return h >= 0 ? -5 : -1;
-    const V3<double> P = axpy(best, d, o);
-    const double margin = 1e-12 * (scale + fabs(o.x) + fabs(o.y) + fabs(o.z) + fabs(P.x) + fabs(P.y) + fabs(P.z));
-    if (second - best <= margin) return -3;
-    V3<double> a, b, e;
-    int f;
-    load_tri_raw64(tris64, win, a, b, e, f);
-    auto inside_axis = [&](double pa, double x0, double x1, double x2) {
-        const double lo = fmin(x0, fmin(x1, x2)), hi = fmax(x0, fmax(x1, x2));
-        return !(hi > lo) || (pa - lo > margin && hi - pa > margin);
-    };
-    if (!inside_axis(P.x, a.x, b.x, e.x) || !inside_axis(P.y, a.y, b.y, e.y) || !inside_axis(P.z, a.z, b.z, e.z))
-        return -4;
-    return win;
-}
-
 // One query of bounce st.k; returns true when the ray has ended (miss or
-// max_bounces reached). A query with an FP32 decision inside its error bound
-// (undecided) is answered in FP64 and sets fell: from its candidates when
-// local is set and they decide it, otherwise by the reference FP64 query on
-// the FP64 tree from the same FP64 ray state (the reference query itself).
+// max_bounces reached). A query whose answer the mixed query cannot vouch for
+// (a near-tie or a hit on the edge of its box) is answered by the reference
+// FP64 query on the FP64 tree from the same FP64 ray state -- the reference
+// query itself -- and sets fell.
 template <int SD, bool CNT, bool REC, bool WW>
 __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<float>& M, const Mesh<double>& M64,
                                                RayState<double>& st, int* sRef, float* sE, Ov<float>::T* ov,
-                                               Ov<double>::T* ov64, int* cIds, float* cLbs, int tid, int32_t* ids,
-                                               Work& w, unsigned long long& queries, unsigned long long& fullQ,
-                                               float K, bool local, bool& fell) {
+                                               Ov<double>::T* ov64, int tid, int32_t* ids, Work& w,
+                                               unsigned long long& queries, unsigned long long& fullQ, float K,
+                                               bool& fell) {
     const V3<double> o = st.o, d = st.d;
-    // FP32 copy of the ray; the launch-plane move only shifts the origin
-    // along the ray, which changes no decision
+    // FP32 copy of the ray for the box tests; the launch-plane move only
+    // shifts the origin along the ray, which changes no decision
     const double shift = st.k == 0 ? J.L0 : 0.0;
     V3<float> of{float(o.x + shift * d.x), float(o.y + shift * d.y), float(o.z + shift * d.z)};
     V3<float> df{float(d.x), float(d.y), float(d.z)};
@@ -862,32 +775,20 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
     V3<float> inv{1.0f / away(df.x), 1.0f / away(df.y), 1.0f / away(df.z)};
     const float ex = K * kEps32 * (fabsf(of.x) + fabsf(of.y) + fabsf(of.z) + 1e-3f);
     const float ed = 2.0f * K * kEps32;
-    float tf;
+    double t64;
     bool amb = false;
-    Cands cand{cIds, cLbs, tid, 0, false};
-    int h = closest_hit_mixed<SD, CNT, WW>(M, of, df, inv, tf, sRef, sE, ov, tid, w, ex, ed, K, cand, amb);
+    int h = closest_hit_mixed<SD, CNT, WW>(M, of, inv, o, d, shift, t64, sRef, sE, ov, tid, w, ex, ed, K, amb);
     if (CNT) queries++;
     // FP64 vertices of the chosen triangle (rtKernels.cu:13-41)
     V3<double> v0, e1, e2;
     int face;
-    int win = -2;
-    if (amb) {
-        fell = true;
-        if (local && !cand.over) {
-            const double R = 1.0 + fmax(fabs(M.rootLo[0]), fabs(M.rootHi[0])) +
-                             fmax(fabs(M.rootLo[1]), fabs(M.rootHi[1])) + fmax(fabs(M.rootLo[2]), fabs(M.rootHi[2]));
-            win = resolve_fp64(M.tris64, o, d, h, cand, R);
-        }
-        if (win == -1) return true;
-    }
-    if (win >= 0 || (!amb && h >= 0)) {
-        load_tri_verts64(M.tris64, win >= 0 ? win : h, v0, e1, e2, face);
-    } else if (!amb) {
-        return true;
+    if (!amb) {
+        if (h < 0) return true;
+        load_tri_verts64(M.tris64, h, v0, e1, e2, face);
     } else {
+        fell = true;
         if (CNT) fullQ++;
         const V3<double> inv64{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
-        double t64;
         const int h64 = closest_hit<double, 0, CNT, WW>(M64, o, d, inv64, t64, nullptr, nullptr, ov64, tid, w);
         if (h64 < 0) return true;
         const TriRec<double> tr = load_tri(static_cast<const double*>(M64.tris), h64);
@@ -923,15 +824,14 @@ template <int SD, bool CNT, bool REC, bool WW>
 __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J, const Job<double>& D,
                                                             const Mesh<float>& M, const Mesh<double>& M64, int i,
                                                             int j, int* sRef, float* sE, Ov<float>::T* ov,
-                                                            Ov<double>::T* ov64, int* cIds, float* cLbs, int tid,
-                                                            int32_t* ids, Work& w, unsigned long long& queries,
-                                                            unsigned long long& fullQ, float K, bool local,
-                                                            bool& fell) {
+                                                            Ov<double>::T* ov64, int tid, int32_t* ids, Work& w,
+                                                            unsigned long long& queries, unsigned long long& fullQ,
+                                                            float K, bool& fell) {
     RayState<double> st;
     ray_init_mixed(D, i, j, st);
     if (J.maxB > 0)
-        while (!ray_step_mixed<SD, CNT, REC, WW>(J, M, M64, st, sRef, sE, ov, ov64, cIds, cLbs, tid, ids, w, queries,
-                                                 fullQ, K, local, fell)) {
+        while (!ray_step_mixed<SD, CNT, REC, WW>(J, M, M64, st, sRef, sE, ov, ov64, tid, ids, w, queries, fullQ, K,
+                                                 fell)) {
         }
     st.P.Ld = st.P.L;
     return st.P;
@@ -980,8 +880,6 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     __shared__ R sE[SD * kWgSize];
     __shared__ Acc<R, MODE> red[kWgSize / 64];
     __shared__ ST sRes[POOL ? 2 * kPoolRays : 1];
-    __shared__ int sCi[MODE == 1 ? kCand * kWgSize : 1];     // mixed: candidate triangles
-    __shared__ float sCl[MODE == 1 ? kCand * kWgSize : 1];   // and their lowest distances
     __shared__ int64_t sNext;
     __shared__ unsigned sPrev;
     __shared__ unsigned sPool;
@@ -1064,8 +962,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                         bool done;
                         if constexpr (MODE == 1) {
                             done = ray_step_mixed<SD, CNT, false, WW>(
-                                J, M, a.meshes64[J.mesh], st, sRef, sE, ov, ov64, sCi, sCl, tid, nullptr, w, cQueries,
-                                cFull, a.errK, a.localFp64 != 0, fell);
+                                J, M, a.meshes64[J.mesh], st, sRef, sE, ov, ov64, tid, nullptr, w, cQueries, cFull,
+                                a.errK, fell);
                         } else {
                             done = ray_step<R, SD, CNT, false, WW>(J, M, st, sRef, sE, ov, tid, nullptr, w,
                                                                    cQueries);
@@ -1128,8 +1026,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     if constexpr (MODE == 1) {
                         const Job<double>& D = a.jobs64[jid];
                         const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW>(
-                            J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, sCi, sCl, tid, ids, w, cQueries,
-                            cFull, a.errK, a.localFp64 != 0, fell);
+                            J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, tid, ids, w, cQueries, cFull,
+                            a.errK, fell);
                         po_term<double>(D, PD, false, dre, dimg);
                         P.count = PD.count;
                         P.Ld = PD.Ld;
