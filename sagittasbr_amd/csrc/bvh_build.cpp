@@ -439,6 +439,45 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
     return true;
 }
 
+template <typename R> std::vector<float> cull_boxes(const Bvh<R>& b, int maxBoxes) {
+    std::vector<int> open{b.root};
+    auto area = [&](int i) {
+        const RefNode<R>& n = b.nodes[i];
+        const double x = double(n.hi[0]) - n.lo[0], y = double(n.hi[1]) - n.lo[1], z = double(n.hi[2]) - n.lo[2];
+        return x * y + y * z + z * x;
+    };
+    for (;;) {
+        int best = -1;
+        double bestA = -1.0;
+        for (size_t q = 0; q < open.size(); ++q)
+            if (b.nodes[open[q]].triCount == 0 && area(open[q]) > bestA) {
+                bestA = area(open[q]);
+                best = int(q);
+            }
+        if (best < 0 || int(open.size()) + 1 > maxBoxes) break;
+        const int n = open[best];
+        open[best] = b.nodes[n].left;
+        open.push_back(b.nodes[n].right);
+    }
+    std::vector<float> out;
+    for (int n : open) {
+        const RefNode<R>& nd = b.nodes[n];
+        for (int k = 0; k < 3; ++k) {
+            float lo = static_cast<float>(nd.lo[k]);
+            if (double(lo) > double(nd.lo[k])) lo = std::nextafter(lo, -INFINITY);
+            out.push_back(lo);
+        }
+        for (int k = 0; k < 3; ++k) {
+            float hi = static_cast<float>(nd.hi[k]);
+            if (double(hi) < double(nd.hi[k])) hi = std::nextafter(hi, INFINITY);
+            out.push_back(hi);
+        }
+    }
+    return out;
+}
+
+template std::vector<float> cull_boxes<float>(const Bvh<float>&, int);
+template std::vector<float> cull_boxes<double>(const Bvh<double>&, int);
 template Bvh<float> build_bvh<float>(const std::vector<float>&, const BuildOptions&);
 template Bvh<double> build_bvh<double>(const std::vector<double>&, const BuildOptions&);
 template bool to_gpu_layout<float>(const Bvh<float>&, GpuBvh<float>&, std::string&);

@@ -74,6 +74,11 @@ struct GpuBvh4 {
 };
 bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err);
 
+// Up to maxBoxes boxes that together hold every triangle: the binary tree's
+// frontier reached by opening the largest-area internal node first. Written
+// as FP32 lo.xyz hi.xyz, rounded outwards.
+template <typename R> std::vector<float> cull_boxes(const Bvh<R>& b, int maxBoxes);
+
 inline int encode_leaf(int firstTri, int count) { return ~((firstTri << 4) | (count - 1)); }
 
 }  // namespace sbr

@@ -87,6 +87,8 @@ template <typename R> struct PrecMesh {
     DevBuf tris64;  // FP32 tree: FP64 vertex records in tree order (80 B each)
     GpuBvh4 gpu4;   // FP32 tree: four-wide collapse (mixed mode)
     DevBuf nodes4;
+    DevBuf cull;    // boxes of the per-tile emptiness test
+    int nCull = 0;
     double buildMs = 0;
 };
 
@@ -167,6 +169,12 @@ template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
     HIP_TRY(&ctx->err, pm.tris.ensure(pm.gpu.tris.size()));
     HIP_TRY(&ctx->err, hipMemcpy(pm.nodes.p, pm.gpu.nodes.data(), pm.gpu.nodes.size(), hipMemcpyHostToDevice));
     HIP_TRY(&ctx->err, hipMemcpy(pm.tris.p, pm.gpu.tris.data(), pm.gpu.tris.size(), hipMemcpyHostToDevice));
+    {
+        const std::vector<float> cb = cull_boxes<R>(pm.bvh, kCullBoxes);
+        pm.nCull = int(cb.size() / 6);
+        HIP_TRY(&ctx->err, pm.cull.ensure(cb.size() * sizeof(float)));
+        HIP_TRY(&ctx->err, hipMemcpy(pm.cull.p, cb.data(), cb.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     if constexpr (std::is_same_v<R, float>) {
         // reference FP64 vertices (rtRcsEntry.cu:136-144 with Real = double)
         // of every triangle in this tree's order, for the mixed mode
@@ -211,6 +219,8 @@ template <typename R> int upload_table(sbr_ctx* ctx) {
             mm.nodes = pm.nodes.p;
             mm.tris = pm.tris.p;
             mm.tris64 = pm.tris64.p;
+            mm.cull = static_cast<const float*>(pm.cull.p);
+            mm.nCull = pm.nCull;
             if constexpr (std::is_same_v<R, float>) {
                 mm.nodes4 = pm.nodes4.p;
                 mm.rootRef4 = pm.gpu4.rootRef;
@@ -326,6 +336,42 @@ bool cull_rect(const Job<R>& J, const R lo[3], const R hi[3], int& i0, int& i1, 
     return true;
 }
 
+// Launch-plane coordinates as linear functionals: with M = [U V D] (columns),
+// [a b c] = M^-1 (X - llc), so a = X . (V x D) / det - llc . (V x D) / det and
+// b = X . (D x U) / det - llc . (D x U) / det (Cramer's rule). Moving llc
+// along D changes neither.
+template <typename R> void launch_coords(Job<R>& J) {
+    double U[3], W[3], D[3], C[3];
+    for (int k = 0; k < 3; ++k) {
+        U[k] = J.uv[k];
+        W[k] = J.vv[k];
+        D[k] = J.dir[k];
+        C[k] = J.llc[k];
+    }
+    auto cross = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double wd[3], du[3];
+    cross(W, D, wd);
+    cross(D, U, du);
+    const double det = U[0] * wd[0] + U[1] * wd[1] + U[2] * wd[2];
+    if (!(std::fabs(det) > 0)) {
+        for (int k = 0; k < 4; ++k) J.cullA[k] = J.cullB[k] = 0.f;
+        return;
+    }
+    double oa = 0, ob = 0;
+    for (int k = 0; k < 3; ++k) {
+        J.cullA[k] = float(wd[k] / det);
+        J.cullB[k] = float(du[k] / det);
+        oa += C[k] * wd[k] / det;
+        ob += C[k] * du[k] / det;
+    }
+    J.cullA[3] = float(oa);
+    J.cullB[3] = float(ob);
+}
+
 constexpr float kErrSafety = 2.0f;  // safety factor on the mixed mode's forward error bounds
 
 template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
@@ -398,6 +444,7 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         J.ny = j.n_y;
         J.maxB = j.max_bounces;
         J.mesh = j.mesh;
+        launch_coords<R>(J);
         int i0 = 0, i1 = j.n_x - 1, j0 = 0, j1 = j.n_y - 1;
         bool any = true;
         if (!record) any = cull_rect<R>(J, pm.gpu.rootLo, pm.gpu.rootHi, i0, i1, j0, j1);

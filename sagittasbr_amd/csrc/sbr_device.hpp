@@ -17,6 +17,7 @@ constexpr int kWideRecBytes = (kWide * 28 + 63) / 64 * 64;  // 6 box floats + 1 
 constexpr int kMaxStack4 = 96;
 constexpr int kMaxDepth4 = kMaxStack4 / (kWide - 1);
 constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a wide record
+constexpr int kCullBoxes = 64;          // boxes of the per-tile emptiness test (one per lane)
 
 // One sigma value (mesh, f, theta, phi) ready for the device. Frame and
 // physics constants are in the working precision R; kd / twoPid carry the
@@ -29,6 +30,9 @@ template <typename R> struct alignas(16) Job {
     float ex0;        // mixed mode: initial ray position error bound [m]
     int nx, ny, maxB, mesh;
     int i0, j0, tilesX, tilesY;      // culled ray rectangle in tiles
+    // launch-plane coordinates of a point X: a = X . cullA[0..2] - cullA[3]
+    // (ray (i, j) sits at a = (i + 0.5) / nx), b likewise with cullB
+    float cullA[4], cullB[4];
     int64_t tileBase;                // first global tile of this job
     int nTiles;
     // Partial sums are kept per 16x16 ray pass on a grid fixed by the culled
@@ -47,6 +51,10 @@ template <typename R> struct alignas(16) Mesh {
     const void* tris64;
     // FP32 tree only: the four-wide collapse of the tree (mixed mode)
     const void* nodes4;
+    // up to kCullBoxes boxes (lo.xyz hi.xyz, FP32, rounded outwards) that
+    // together hold every triangle, for the per-tile emptiness test
+    const float* cull;
+    int nCull;
     R rootLo[3], rootHi[3];
     int rootRef;
     int rootRef4;

@@ -837,6 +837,39 @@ __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J,
     return st.P;
 }
 
+// Whether some ray of the ray rectangle [ia, ib] x [ja, jb] of a job can
+// reach one of the mesh's cull boxes, which together hold every triangle
+// (lane q tests box q; the result is wave-uniform). A box is reached only by
+// rays whose launch-plane coordinates fall in its projection along the ray
+// direction; the test compares that projection's bounding rectangle, widened
+// by half a ray spacing and 1e-5 of the grid (far above its FP32 rounding),
+// with the rectangle of ray centres. A tile none of whose rays can reach a
+// box contributes exactly zero and is not traced.
+template <typename R>
+__device__ __forceinline__ bool tile_reaches_mesh(const Job<R>& J, const Mesh<R>& M, int ia, int ib, int ja, int jb,
+                                                  int lane) {
+    bool reach = false;
+    if (lane < M.nCull) {
+        gptr<float> c = as_global<float>(M.cull) + 6 * lane;
+        float amin = -J.cullA[3], amax = -J.cullA[3], bmin = -J.cullB[3], bmax = -J.cullB[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float lo = c[k], hi = c[3 + k];
+            const float pa = lo * J.cullA[k], qa = hi * J.cullA[k];
+            const float pb = lo * J.cullB[k], qb = hi * J.cullB[k];
+            amin += fminf(pa, qa);
+            amax += fmaxf(pa, qa);
+            bmin += fminf(pb, qb);
+            bmax += fmaxf(pb, qb);
+        }
+        const float ma = 1e-5f + 0.5f / float(J.nx), mb = 1e-5f + 0.5f / float(J.ny);
+        const float aLo = (float(ia) + 0.5f) / float(J.nx), aHi = (float(ib) + 0.5f) / float(J.nx);
+        const float bLo = (float(ja) + 0.5f) / float(J.ny), bHi = (float(jb) + 0.5f) / float(J.ny);
+        reach = amax >= aLo - ma && amin <= aHi + ma && bmax >= bLo - mb && bmin <= bHi + mb;
+    }
+    return __ballot(reach) != 0ull;
+}
+
 // Per-ray result slot of the ray-pool variant: the ray's (re, im) term in
 // the precision the accumulator adds it in.
 template <typename R, int MODE> struct SlotOf {
@@ -920,6 +953,13 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
         const int tl = static_cast<int>(tile - J.tileBase);
         const int ts = a.tileSide;
         const int tx = tl % J.tilesX, ty = tl / J.tilesX;
+        // tiles none of whose rays can reach the mesh are not traced (not in
+        // record or counting runs, which account for every ray)
+        bool skip = false;
+        if constexpr (!REC && !CNT) {
+            const int ia = J.i0 + tx * ts, ja = J.j0 + ty * ts;
+            skip = !tile_reaches_mesh(J, M, ia, min(ia + ts - 1, J.nx - 1), ja, min(ja + ts - 1, J.ny - 1), lane);
+        }
 
         using AccT = Acc<R, MODE>;
         AccT* parts = static_cast<AccT*>(a.partials);
@@ -950,7 +990,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 RayState<RS> st;
                 Work w{0, 0};
                 bool fell = false;
-                bool alive = fetch();
+                bool alive = skip ? false : fetch();
                 auto start = [&]() {
                     if (CNT) cRays++;
                     if constexpr (MODE == 1) ray_init_mixed(a.jobs64[jid], ri, rj, st);
@@ -990,7 +1030,14 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     cVisits += w.visits;
                     cTests += w.tests;
                 }
-                poolBase += nRays + kWgSize;
+                if (skip) {
+                    for (unsigned q = tid; q < nRays; q += kWgSize) {
+                        sRes[2 * q] = ST(0);
+                        sRes[2 * q + 1] = ST(0);
+                    }
+                } else {
+                    poolBase += nRays + kWgSize;
+                }
                 __syncthreads();
                 for (int pp = 0; pp < nb; ++pp) {
                     AccT acc = AccT::zero();
@@ -1014,7 +1061,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 const int i = J.i0 + tx * ts + px + qx;
                 const int j = J.j0 + ty * ts + py + qy;
                 do {
-                    if (i >= J.nx || j >= J.ny) break;
+                    if (skip || i >= J.nx || j >= J.ny) break;
                     if (CNT) cRays++;
                     const int ray = j * J.nx + i;
                     int32_t* ids = REC ? a.recIds + size_t(ray) * J.maxB : nullptr;

@@ -326,6 +326,33 @@ def test_autotune(ctx, tri_duct):
         pl.close()
 
 
+@pytest.mark.parametrize("body", [2, 5])
+def test_per_ray_mixed_cavity_bodies(ctx, body):
+    """Mixed vs FP64 per ray on cylinders with a rectangular cavity (the
+    random-bodies workload's multi-bounce kind), three angles each: hit
+    faces, bounce counts and path lengths identical."""
+    bodies = M.random_bodies(2, 6)
+    b = bodies[body]
+    assert b["kind"] == "cavity"
+    mesh = ctx.mesh(b["verts"], b["faces"])
+    r = float(np.max(np.linalg.norm(b["verts"], axis=1)))
+    g = 2.02 * r
+    n = int(np.ceil(g / (299792458.0 / b["freq"] / 10)))
+    flagged = 0
+    for th, ph in ((90.0, 0.0), (70.0, 33.0), (90.0, 90.0)):
+        p = S.SweepParams.make(theta_start=th, theta_end=th, theta_samples=1, phi_start=ph, phi_end=ph, phi_samples=1,
+                               grid_size=g, n_x=n, n_y=n, max_bounces=6, freq=b["freq"])
+        _step(f"cavity body {body} ({len(b['faces'])} triangles, {n}x{n} rays) theta {th} phi {ph}")
+        gm = ctx.trace_rays(mesh, p, 0, S.SBR_MIXED)
+        g6 = ctx.trace_rays(mesh, p, 0, S.SBR_FP64)
+        assert np.array_equal(gm["count"], g6["count"])
+        assert np.array_equal(gm["ids"], g6["ids"])
+        assert np.array_equal(gm["L"], g6["L"])
+        flagged += int(gm["retraced"].sum())
+    print(f"    rays with an FP64-answered query: {flagged}", flush=True)
+    ctx.destroy_mesh(mesh)
+
+
 @pytest.mark.parametrize("theta,phi", [(92.0, 223.0), (88.0, 230.0), (90.0, 242.0), (60.0, 217.0), (100.0, 240.0)])
 def test_mixed_airframe_fin_angles(ctx, theta, phi):
     """Regression: on the airframe's fin two triangles are hit 8e-5 m apart
