@@ -437,8 +437,8 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     float cx = 0.5f * (m.rootLo[0] + m.rootHi[0]), cy = 0.5f * (m.rootLo[1] + m.rootHi[1]),
           cz = 0.5f * (m.rootLo[2] + m.rootHi[2]);
     float hx = m.rootHi[0] - cx, hy = m.rootHi[1] - cy, hz = m.rootHi[2] - cz;
-    float tFar = sqrtf((o.x - cx) * (o.x - cx) + (o.y - cy) * (o.y - cy) + (o.z - cz) * (o.z - cz)) +
-                 sqrtf(hx * hx + hy * hy + hz * hz);
+    // (L1 norms bound the Euclidean ones)
+    float tFar = (fabsf(o.x - cx) + fabsf(o.y - cy) + fabsf(o.z - cz) + hx + hy + hz) * (1.0f + 4.0f * kEps32);
     // position and distance error bounds of the FP32 box tests over the query
     float dx = ex + ed * tFar + K * kEps32 * (o1 + tFar);
     float tErr = dx + 4.0f * K * kEps32 * tFar;
@@ -963,7 +963,15 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
 
         using AccT = Acc<R, MODE>;
         AccT* parts = static_cast<AccT*>(a.partials);
-        if constexpr (POOL) {
+        if (skip) {
+            // every pass of the tile sums to zero: store the zero partials
+            // directly (skip is the same for the whole workgroup)
+            const int perSide = ts >> 4;
+            if (tid < perSide * perSide) {
+                const int sx = (tx * ts + (tid % perSide) * 16) >> 4, sy = (ty * ts + (tid / perSide) * 16) >> 4;
+                if (sx < J.passX && sy < J.passY) parts[J.passBase + int64_t(sy) * J.passX + sx] = AccT::zero();
+            }
+        } else if constexpr (POOL) {
             const int perSide = ts >> 4;
             const int nPass = perSide * perSide;
             for (int b0 = 0; b0 < nPass; b0 += kPoolRays / 256) {
@@ -990,7 +998,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 RayState<RS> st;
                 Work w{0, 0};
                 bool fell = false;
-                bool alive = skip ? false : fetch();
+                bool alive = fetch();
                 auto start = [&]() {
                     if (CNT) cRays++;
                     if constexpr (MODE == 1) ray_init_mixed(a.jobs64[jid], ri, rj, st);
@@ -1030,14 +1038,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     cVisits += w.visits;
                     cTests += w.tests;
                 }
-                if (skip) {
-                    for (unsigned q = tid; q < nRays; q += kWgSize) {
-                        sRes[2 * q] = ST(0);
-                        sRes[2 * q + 1] = ST(0);
-                    }
-                } else {
-                    poolBase += nRays + kWgSize;
-                }
+                poolBase += nRays + kWgSize;
                 __syncthreads();
                 for (int pp = 0; pp < nb; ++pp) {
                     AccT acc = AccT::zero();
@@ -1061,7 +1062,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 const int i = J.i0 + tx * ts + px + qx;
                 const int j = J.j0 + ty * ts + py + qy;
                 do {
-                    if (skip || i >= J.nx || j >= J.ny) break;
+                    if (i >= J.nx || j >= J.ny) break;
                     if (CNT) cRays++;
                     const int ray = j * J.nx + i;
                     int32_t* ids = REC ? a.recIds + size_t(ray) * J.maxB : nullptr;
