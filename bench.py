@@ -253,10 +253,15 @@ def other_workload(args):
     rays = sum(j.n_x * j.n_y for j in jm)
     plan = ctx.plan(jm, S.SBR_MIXED, S.SBR_BATCHED)
     log(f"{n} jobs, {rays:.3e} launched rays; autotuning")
-    # the airframe's candidates differ by a few percent: two timed runs each
-    # on a larger sample keep the choice stable
-    tuning = plan.autotune(sample_jobs=min(n, 160 if args.workload == "c3" else 512),
-                           repeats=2 if args.workload == "c3" else 1)
+    if args.tuning:
+        tuning = json.loads(args.tuning)
+        plan.tune(tuning["wg_per_cu"], tuning["lds_stack"], tuning["tile_rays"], tuning["use_graph"],
+                  tuning.get("ray_pool", 0), tuning.get("traversal", 0))
+    else:
+        # the airframe's candidates differ by a few percent: two timed runs
+        # each on a larger sample keep the choice stable
+        tuning = plan.autotune(sample_jobs=min(n, 160 if args.workload == "c3" else 512),
+                               repeats=2 if args.workload == "c3" else 1)
     if args.workload == "c4":
         plan.tune(tuning["wg_per_cu"], tuning["lds_stack"], tuning["tile_rays"], 1, tuning["ray_pool"],
                   tuning["traversal"])
@@ -315,7 +320,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sweep would shard by angle "
-              "tiles across GPUs, see DESIGN.md", file=sys.stderr)
+              "tiles across GPUs (not built: one GPU is measured)", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
