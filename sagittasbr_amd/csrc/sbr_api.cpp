@@ -288,6 +288,7 @@ template <typename R> void make_phys(const sbr_job& j, Job<R>& J) {
     R g = R(j.grid_size);
     R area = (g * g) / R(j.n_x * j.n_y);
     J.twoPi = R(2.0) * R(M_PI);
+    J.invTwoPi = R(1.0) / J.twoPi;
     J.pre = (J.k * area / (R(4.0) * R(M_PI))) * R(2.0);
     J.refl = R(j.reflection_const);
     J.kd = 2.0 * M_PI / (299792458.0 / j.freq);
@@ -432,6 +433,7 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
             J.k = R(D.k);
             J.pre = R(D.pre);
             J.twoPi = R(D.twoPi);
+            J.invTwoPi = R(D.invTwoPi);
             J.refl = R(D.refl);
             J.kd = D.kd;
             J.twoPid = D.twoPid;

@@ -25,6 +25,7 @@ constexpr int kCullBoxes = 64;          // boxes of the per-tile emptiness test 
 template <typename R> struct alignas(16) Job {
     R llc[3], uv[3], vv[3], dir[3];  // launch frame, sweep.cu:349-364
     R k, pre, twoPi, refl;           // k, (k*dA/4pi)*2, 2pi, reflection const
+    R invTwoPi;                      // 1 / twoPi rounded (phase reduction's quotient estimate)
     double kd, twoPid;
     double L0;        // mixed mode: distance the launch plane was moved towards the body
     float ex0;        // mixed mode: initial ray position error bound [m]
@@ -47,7 +48,7 @@ template <typename R> struct alignas(16) Mesh {
     const void* tris;
     // FP32 tree only: the reference FP64 vertices of each triangle, in the
     // FP32 tree's order (v0 v1 v2 as 9 doubles + face id), for the mixed
-    // mode's FP64 recompute of a hit
+    // mode's FP64 triangle tests
     const void* tris64;
     // FP32 tree only: the four-wide collapse of the tree (mixed mode)
     const void* nodes4;

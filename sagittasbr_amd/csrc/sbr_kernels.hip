@@ -27,14 +27,14 @@
 // file is compiled with -ffp-contract=off so nothing else is fused. Division
 // and square root are IEEE correctly rounded.
 //
-// Mixed mode (MODE 1): FP32 traversal whose every relevant decision carries
-// a forward error bound against the FP64 reference ray (filtered
-// predicates). Boxes are tested grown by the ray's position error bound, so
-// pruning is conservative; a query with a Moller-Trumbore decision whose
-// barycentric, determinant or distance margin falls inside the bound is
-// answered instead by the reference FP64 query on the FP64 tree, from the
-// same FP64 ray state. Every query therefore has the FP64 outcome; the ray
-// state, path length and phase are FP64 and the complex sum is FP32
+// Mixed mode (MODE 1): FP32 box traversal of a four-wide tree against an
+// FP32 copy of the FP64 reference ray, every box grown by that copy's
+// position error bound (so pruning is conservative), and the reference FP64
+// Moller-Trumbore test of every triangle reached. A query whose answer could
+// differ from the reference FP64 query's (a near-tie, or a hit on the edge
+// of its triangle's box) is answered by that query itself on the FP64 tree,
+// from the same FP64 ray state. Every query therefore has the FP64 outcome;
+// the ray state, path length and phase are FP64 and the complex sum is FP32
 // two-sum compensated.
 #include <hip/hip_runtime.h>
 
@@ -141,17 +141,6 @@ __device__ __forceinline__ TriRec<float> load_tri(const float* tris, int i) {
     vf4 a = p[0], b = p[1], c = p[2];
     V3<float> v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
     return {v0, sub(v1, v0), sub(v2, v0), __float_as_int(c.y), c.z, c.w};
-}
-// Reference FP64 vertices of FP32-tree triangle i (mixed mode's recompute);
-// the edges are formed as rtKernels.cu:13-14 forms them, in FP64.
-__device__ __forceinline__ void load_tri_verts64(const void* tris64, int i, V3<double>& v0, V3<double>& e1,
-                                                 V3<double>& e2, int& face) {
-    gptr<vd2> p = as_global<vd2>(tris64) + 5 * i;
-    vd2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
-    v0 = {a.x, a.y, b.x};
-    e1 = {b.y - v0.x, c.x - v0.y, c.y - v0.z};
-    e2 = {d.x - v0.x, d.y - v0.y, e.x - v0.z};
-    face = static_cast<int>(__double_as_longlong(e.y));
 }
 // The same record's three vertices as stored, and the face id.
 __device__ __forceinline__ void load_tri_raw64(const void* tris64, int i, V3<double>& v0, V3<double>& v1,
@@ -426,12 +415,14 @@ constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
 // position units (R: the mesh's largest coordinate magnitude per axis,
 // summed), so the margin is 1e-12 of that. The FP32 origin may be moved
 // along the ray by shift (first query): FP32 box distances are then shift
-// shorter than the FP64 ones.
+// shorter than the FP64 ones. Unless amb is set, a hit's FP64 vertices and
+// face id are returned in hv0, hv1, hv2 and hFace.
 template <int SD, bool CNT, bool WW>
 __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> inv, V3<double> o64,
                                                  V3<double> d64, double shift, double& tHit, int* sRef, float* sE,
                                                  Ov<float>::T* ov, int tid, Work& w, float ex, float ed, float K,
-                                                 bool& amb) {
+                                                 bool& amb, V3<double>& hv0, V3<double>& hv1, V3<double>& hv2,
+                                                 int& hFace) {
     const float o1 = fabsf(o.x) + fabsf(o.y) + fabsf(o.z);
     // far distance of anything in the root box from the origin
     float cx = 0.5f * (m.rootLo[0] + m.rootHi[0]), cy = 0.5f * (m.rootLo[1] + m.rootHi[1]),
@@ -543,15 +534,13 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
         if (second - closest <= margin) {
             amb = true;
         } else {
-            V3<double> a, b, c;
-            int f;
-            load_tri_raw64(m.tris64, hit, a, b, c, f);
+            load_tri_raw64(m.tris64, hit, hv0, hv1, hv2, hFace);
             auto inside_axis = [&](double pa, double x0, double x1, double x2) {
                 const double lo = fmin(x0, fmin(x1, x2)), hi = fmax(x0, fmax(x1, x2));
                 return !(hi > lo) || (pa - lo > margin && hi - pa > margin);
             };
-            amb = !inside_axis(P.x, a.x, b.x, c.x) || !inside_axis(P.y, a.y, b.y, c.y) ||
-                  !inside_axis(P.z, a.z, b.z, c.z);
+            amb = !inside_axis(P.x, hv0.x, hv1.x, hv2.x) || !inside_axis(P.y, hv0.y, hv1.y, hv2.y) ||
+                  !inside_axis(P.z, hv0.z, hv1.z, hv2.z);
         }
     }
     tHit = closest;
@@ -718,10 +707,85 @@ __device__ __forceinline__ PathOut<R> trace_path(const Job<R>& J, const Mesh<R>&
     return st.P;
 }
 
+// fmod(x, y) for x >= 0 and y > 0 given invY ~ 1 / y: the quotient from one
+// multiply by invY is off by at most one below the size limit and is
+// corrected, and the remainder x - n y is formed by one fused multiply-add,
+// exact for the correct n (the remainder of a floating-point division is
+// representable), so the value is fmod's. Larger quotients go to fmod.
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return fma(a, b, c); }
+template <typename R> __device__ __forceinline__ R fmod_pos(R x, R y, R invY) {
+    constexpr R kMaxQ = std::is_same_v<R, double> ? R(1ull << 50) : R(1 << 20);
+    R n = floor(x * invY);
+    if (!(n < kMaxQ)) return fmodr(x, y);
+    R r = fma_r(-n, y, x);
+    if (r < R(0)) {
+        n -= R(1);
+        r = fma_r(-n, y, x);
+    } else if (r >= y) {
+        n += R(1);
+        r = fma_r(-n, y, x);
+    }
+    return r;
+}
+
+// sin and cos of x in [0, 2 pi] in FP64 (the mixed mode's reduced phase).
+// Cody-Waite reduction by the quadrant q = rint(2 x / pi), with pi / 2 split
+// into two doubles, leaves |r| <= pi / 4 (+ 1e-16); there the Taylor series
+// of sin to r^17 and of cos to r^16 are truncated below 1e-19, so the error
+// is the FP64 evaluation's, about one unit in the last place of 1. Shorter
+// than the general sincos, which also carries the large-argument reduction.
+// An FP64 constant with a nonzero low word takes two registers. The compiler
+// moves such constants out of the ray loops, where they stay live across the
+// traversal and are spilled to scratch and reloaded every bounce; passed
+// through an empty volatile asm they are formed at their use, two scalar
+// moves each, on the scalar unit.
+__device__ __forceinline__ double at_use(double c) {
+    asm volatile("" : "+s"(c));
+    return c;
+}
+
+__device__ __forceinline__ void sincos_2pi(double x, double* s, double* c) {
+    constexpr double kTwoOverPi = 0.63661977236758134308;
+    constexpr double kPio2Hi = 1.5707963267948966;     // pi / 2 rounded
+    constexpr double kPio2Lo = 6.123233995736766e-17;  // pi / 2 - kPio2Hi
+    const double q = rint(x * at_use(kTwoOverPi));
+    double r = fma(-q, at_use(kPio2Hi), x);
+    r = fma(-q, at_use(kPio2Lo), r);
+    const double z = r * r;
+    // sin r = r + r^3 (-1/3! + z (1/5! - z (1/7! - ...))), Horner in z
+    double ps = at_use(1.0 / 355687428096000.0);  // 1/17!
+    ps = fma(ps, z, at_use(-1.0 / 1307674368000.0));
+    ps = fma(ps, z, at_use(1.0 / 6227020800.0));
+    ps = fma(ps, z, at_use(-1.0 / 39916800.0));
+    ps = fma(ps, z, at_use(1.0 / 362880.0));
+    ps = fma(ps, z, at_use(-1.0 / 5040.0));
+    ps = fma(ps, z, at_use(1.0 / 120.0));
+    ps = fma(ps, z, at_use(-1.0 / 6.0));
+    const double sr = fma(r * z, ps, r);
+    // cos r = 1 + z (-1/2! + z (1/4! - ...))
+    double pc = at_use(1.0 / 20922789888000.0);  // 1/16!
+    pc = fma(pc, z, at_use(-1.0 / 87178291200.0));
+    pc = fma(pc, z, at_use(1.0 / 479001600.0));
+    pc = fma(pc, z, at_use(-1.0 / 3628800.0));
+    pc = fma(pc, z, at_use(1.0 / 40320.0));
+    pc = fma(pc, z, at_use(-1.0 / 720.0));
+    pc = fma(pc, z, at_use(1.0 / 24.0));
+    pc = fma(pc, z, at_use(-0.5));
+    const double cr = fma(z, pc, 1.0);
+    // x = q pi/2 + r: rotate by the quadrant
+    const int n = static_cast<int>(q) & 3;
+    const double a = (n & 1) ? cr : sr, b = (n & 1) ? sr : cr;
+    *s = (n & 2) ? -a : a;
+    *c = ((n + 1) & 2) ? -b : b;
+}
+
 // PO term (poKernels.cu:75-110) from a finished path; the phase uses the
-// FP64 path length when fp64Phase is set.
-template <typename R>
+// FP64 path length when fp64Phase is set. RED (FP64 only): sin and cos of
+// the reduced phase by sincos_2pi.
+template <typename R, bool RED = false>
 __device__ __forceinline__ void po_term(const Job<R>& J, const PathOut<R>& P, bool fp64Phase, R& re, R& im) {
+    static_assert(!RED || std::is_same_v<R, double>, "reduced-phase sincos is FP64");
     re = R(0);
     im = R(0);
     if (P.count <= 0) return;
@@ -731,10 +795,11 @@ __device__ __forceinline__ void po_term(const Job<R>& J, const PathOut<R>& P, bo
     if (!(cosT > R(0))) return;
     R ph;
     if (fp64Phase) ph = static_cast<R>(fmodr((2.0 * J.kd) * P.Ld, J.twoPid));
-    else ph = fmodr((R(2.0) * J.k) * P.L, J.twoPi);
+    else ph = fmod_pos((R(2.0) * J.k) * P.L, J.twoPi, J.invTwoPi);
     R mag = (J.pre * cosT) * refl;
     R sn, cs;
-    sincosr(ph, &sn, &cs);
+    if constexpr (RED) sincos_2pi(ph, &sn, &cs);
+    else sincosr(ph, &sn, &cs);
     re = -mag * sn;
     im = mag * cs;
 }
@@ -743,8 +808,7 @@ __device__ __forceinline__ void po_term(const Job<R>& J, const PathOut<R>& P, bo
 // normal) is carried in FP64 with the reference's FP64 operations, so it is
 // the FP64 reference ray itself; each closest-hit query runs on an FP32 copy
 // of it (the first one from a launch plane moved next to the body) with
-// error-bounded decisions, and the chosen triangle's distance is then
-// recomputed in FP64.
+// error-bounded box decisions and FP64 triangle tests.
 __device__ __forceinline__ void ray_init_mixed(const Job<double>& D, int i, int j, RayState<double>& st) {
     st.o = ray_origin(D, i, j);
     st.d = V3<double>{D.dir[0], D.dir[1], D.dir[2]};
@@ -775,36 +839,33 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
     V3<float> inv{1.0f / away(df.x), 1.0f / away(df.y), 1.0f / away(df.z)};
     const float ex = K * kEps32 * (fabsf(of.x) + fabsf(of.y) + fabsf(of.z) + 1e-3f);
     const float ed = 2.0f * K * kEps32;
-    double t64;
+    double t;
     bool amb = false;
-    int h = closest_hit_mixed<SD, CNT, WW>(M, of, inv, o, d, shift, t64, sRef, sE, ov, tid, w, ex, ed, K, amb);
-    if (CNT) queries++;
-    // FP64 vertices of the chosen triangle (rtKernels.cu:13-41)
-    V3<double> v0, e1, e2;
+    V3<double> v0, v1, v2, e1, e2;
     int face;
+    int h = closest_hit_mixed<SD, CNT, WW>(M, of, inv, o, d, shift, t, sRef, sE, ov, tid, w, ex, ed, K, amb, v0, v1,
+                                           v2, face);
+    if (CNT) queries++;
+    // FP64 edges of the chosen triangle (rtKernels.cu:13-14)
     if (!amb) {
         if (h < 0) return true;
-        load_tri_verts64(M.tris64, h, v0, e1, e2, face);
+        e1 = sub(v1, v0);
+        e2 = sub(v2, v0);
     } else {
         fell = true;
         if (CNT) fullQ++;
         const V3<double> inv64{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
-        const int h64 = closest_hit<double, 0, CNT, WW>(M64, o, d, inv64, t64, nullptr, nullptr, ov64, tid, w);
+        const int h64 = closest_hit<double, 0, CNT, WW>(M64, o, d, inv64, t, nullptr, nullptr, ov64, tid, w);
         if (h64 < 0) return true;
         const TriRec<double> tr = load_tri(static_cast<const double*>(M64.tris), h64);
-        v0 = tr.v0;
         e1 = tr.e1;
         e2 = tr.e2;
         face = tr.face;
     }
-    // FP64 distance and bounce update, as the FP64 query and ray_step form
-    // them (the FP64 tree's edges equal v1 - v0, v2 - v0 of these vertices)
-    V3<double> p = cross3(d, e2);
-    double det = dot3(e1, p);
-    double invd = 1.0 / det;
-    V3<double> tv = sub(o, v0);
-    V3<double> q = cross3(tv, e1);
-    double t = dot3(e2, q) * invd;
+    // bounce update as ray_step forms it in FP64. t is the FP64
+    // Moller-Trumbore distance of that triangle (both queries form it with
+    // mt_test<double>; the FP64 tree's edges equal v1 - v0, v2 - v0 of the
+    // same vertices)
     V3<double> n = unit3(cross3(e1, e2));
     V3<double> hp = axpy(t, d, o);
     st.P.cosT = dot3(n, V3<double>{-d.x, -d.y, -d.z});
@@ -1020,7 +1081,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                             ST re = ST(0), im = ST(0);
                             if constexpr (MODE == 1) {
                                 st.P.Ld = st.P.L;
-                                po_term<double>(a.jobs64[jid], st.P, false, re, im);
+                                po_term<double, true>(a.jobs64[jid], st.P, false, re, im);
                                 if (CNT && fell) cRetrace++;
                                 fell = false;
                             } else {
@@ -1076,7 +1137,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                         const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW>(
                             J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, tid, ids, w, cQueries, cFull,
                             a.errK, fell);
-                        po_term<double>(D, PD, false, dre, dimg);
+                        po_term<double, true>(D, PD, false, dre, dimg);
                         P.count = PD.count;
                         P.Ld = PD.Ld;
                     } else {
