@@ -329,16 +329,9 @@ template <typename R> bool to_gpu_layout(const Bvh<R>& b, GpuBvh<R>& out, std::s
         int face = b.perm[t];
         if (sizeof(R) == 4) {
             // FP32: the three vertices (edges are formed per test, as the
-            // reference does, and stay exact for the FP64 recompute of the
-            // mixed mode), plus |e1| + |e2| and |e1| |e2| for its error bounds
+            // reference does) and the face id; the last 8 bytes stay zero
             std::memcpy(p, v, 9 * sizeof(R));
             std::memcpy(p + 9 * sizeof(R), &face, sizeof(int));
-            double e1[3] = {double(v[3]) - v[0], double(v[4]) - v[1], double(v[5]) - v[2]};
-            double e2[3] = {double(v[6]) - v[0], double(v[7]) - v[1], double(v[8]) - v[2]};
-            double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
-            double l2 = std::sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
-            float pads[2] = {float((l1 + l2) * (1 + 1e-6)), float(l1 * l2 * (1 + 1e-6))};
-            std::memcpy(p + 9 * sizeof(R) + sizeof(int), pads, sizeof(pads));
         } else {
             // FP64: v0, e1 = v1 - v0, e2 = v2 - v0 (exact for float inputs)
             R rec9[9] = {v[0], v[1], v[2], v[3] - v[0], v[4] - v[1], v[5] - v[2], v[6] - v[0], v[7] - v[1], v[8] - v[2]};

@@ -131,16 +131,15 @@ typedef int vi4 __attribute__((ext_vector_type(4)));
 template <typename R> struct TriRec {
     V3<R> v0, e1, e2;
     int face;
-    float sumE, prodE;  // |e1| + |e2| and |e1| |e2| (FP32 layout only)
 };
 
-// FP32 record: v0, v1, v2, face, |e1|+|e2|, |e1||e2|; the edges are formed
+// FP32 record: v0, v1, v2, face, 8 bytes of padding; the edges are formed
 // here exactly as rtKernels.cu:13-14 forms them.
 __device__ __forceinline__ TriRec<float> load_tri(const float* tris, int i) {
     gptr<vf4> p = as_global<vf4>(tris) + 3 * i;
     vf4 a = p[0], b = p[1], c = p[2];
     V3<float> v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
-    return {v0, sub(v1, v0), sub(v2, v0), __float_as_int(c.y), c.z, c.w};
+    return {v0, sub(v1, v0), sub(v2, v0), __float_as_int(c.y)};
 }
 // The same record's three vertices as stored, and the face id.
 __device__ __forceinline__ void load_tri_raw64(const void* tris64, int i, V3<double>& v0, V3<double>& v1,
@@ -156,7 +155,7 @@ __device__ __forceinline__ TriRec<double> load_tri(const double* tris, int i) {
     gptr<vd2> p = as_global<vd2>(tris) + 5 * i;
     vd2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
     int face = static_cast<int>(__double_as_longlong(e.y) & 0xffffffffll);
-    return {{a.x, a.y, b.x}, {b.y, c.x, c.y}, {d.x, d.y, e.x}, face, 0.f, 0.f};
+    return {{a.x, a.y, b.x}, {b.y, c.x, c.y}, {d.x, d.y, e.x}, face};
 }
 
 template <typename R> struct NodeRec {
@@ -489,7 +488,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
             V3<double> a, b, c;
             int f;
             load_tri_raw64(m.tris64, first + k, a, b, c, f);
-            const TriRec<double> tr{a, sub(b, a), sub(c, a), f, 0.f, 0.f};
+            const TriRec<double> tr{a, sub(b, a), sub(c, a), f};
             double t;
             if (mt_test(tr, o64, d64, 1e20, t)) {
                 if (t < closest) {
