@@ -229,6 +229,9 @@ template <typename R> int upload_table(sbr_ctx* ctx) {
                 mm.rootLo[k] = pm.gpu.rootLo[k];
                 mm.rootHi[k] = pm.gpu.rootHi[k];
             }
+            mm.tieScale = 1.0;
+            for (int k = 0; k < 3; ++k)
+                mm.tieScale += std::fmax(std::fabs(double(mm.rootLo[k])), std::fabs(double(mm.rootHi[k])));
             mm.rootRef = pm.gpu.rootRef;
             mm.nTris = static_cast<int>(pm.bvh.perm.size());
         }

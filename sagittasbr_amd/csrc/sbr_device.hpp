@@ -21,7 +21,7 @@ constexpr int kCullBoxes = 64;          // boxes of the per-tile emptiness test 
 
 // One sigma value (mesh, f, theta, phi) ready for the device. Frame and
 // physics constants are in the working precision R; kd / twoPid carry the
-// FP64 values used by the mixed-precision phase.
+// FP64 wavenumber and 2 pi for the adjoint.
 template <typename R> struct alignas(16) Job {
     R llc[3], uv[3], vv[3], dir[3];  // launch frame, sweep.cu:349-364
     R k, pre, twoPi, refl;           // k, (k*dA/4pi)*2, 2pi, reflection const
@@ -57,6 +57,9 @@ template <typename R> struct alignas(16) Mesh {
     const float* cull;
     int nCull;
     R rootLo[3], rootHi[3];
+    // 1 + the largest coordinate magnitude of the root box per axis, summed
+    // (mixed mode's tie margin scale)
+    double tieScale;
     int rootRef;
     int rootRef4;
     int nTris;

@@ -524,9 +524,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
         }
     }
     if (hit >= 0) {
-        const double R = 1.0 + fmax(fabs(double(m.rootLo[0])), fabs(double(m.rootHi[0]))) +
-                         fmax(fabs(double(m.rootLo[1])), fabs(double(m.rootHi[1]))) +
-                         fmax(fabs(double(m.rootLo[2])), fabs(double(m.rootHi[2])));
+        const double R = m.tieScale;
         const V3<double> P = axpy(closest, d64, o64);
         const double margin =
             1e-12 * (R + fabs(o64.x) + fabs(o64.y) + fabs(o64.z) + fabs(P.x) + fabs(P.y) + fabs(P.z));
@@ -779,11 +777,10 @@ __device__ __forceinline__ void sincos_2pi(double x, double* s, double* c) {
     *c = ((n + 1) & 2) ? -b : b;
 }
 
-// PO term (poKernels.cu:75-110) from a finished path; the phase uses the
-// FP64 path length when fp64Phase is set. RED (FP64 only): sin and cos of
-// the reduced phase by sincos_2pi.
+// PO term (poKernels.cu:75-110) from a finished path. RED (FP64 only): sin
+// and cos of the reduced phase by sincos_2pi.
 template <typename R, bool RED = false>
-__device__ __forceinline__ void po_term(const Job<R>& J, const PathOut<R>& P, bool fp64Phase, R& re, R& im) {
+__device__ __forceinline__ void po_term(const Job<R>& J, const PathOut<R>& P, R& re, R& im) {
     static_assert(!RED || std::is_same_v<R, double>, "reduced-phase sincos is FP64");
     re = R(0);
     im = R(0);
@@ -792,9 +789,7 @@ __device__ __forceinline__ void po_term(const Job<R>& J, const PathOut<R>& P, bo
     for (int b = 0; b < P.count; ++b) refl *= J.refl;
     const R cosT = P.cosT;
     if (!(cosT > R(0))) return;
-    R ph;
-    if (fp64Phase) ph = static_cast<R>(fmodr((2.0 * J.kd) * P.Ld, J.twoPid));
-    else ph = fmod_pos((R(2.0) * J.k) * P.L, J.twoPi, J.invTwoPi);
+    const R ph = fmod_pos((R(2.0) * J.k) * P.L, J.twoPi, J.invTwoPi);
     R mag = (J.pre * cosT) * refl;
     R sn, cs;
     if constexpr (RED) sincos_2pi(ph, &sn, &cs);
@@ -1080,11 +1075,11 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                             ST re = ST(0), im = ST(0);
                             if constexpr (MODE == 1) {
                                 st.P.Ld = st.P.L;
-                                po_term<double, true>(a.jobs64[jid], st.P, false, re, im);
+                                po_term<double, true>(a.jobs64[jid], st.P, re, im);
                                 if (CNT && fell) cRetrace++;
                                 fell = false;
                             } else {
-                                po_term<R>(J, st.P, false, re, im);
+                                po_term<R>(J, st.P, re, im);
                             }
                             if (CNT) cBounces += st.P.count;
                             sRes[2 * r] = re;
@@ -1136,12 +1131,12 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                         const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW>(
                             J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, tid, ids, w, cQueries, cFull,
                             a.errK, fell);
-                        po_term<double, true>(D, PD, false, dre, dimg);
+                        po_term<double, true>(D, PD, dre, dimg);
                         P.count = PD.count;
                         P.Ld = PD.Ld;
                     } else {
                         P = trace_path<R, SD, CNT, REC, WW>(J, M, i, j, sRef, sE, ov, tid, ids, w, cQueries);
-                        po_term<R>(J, P, false, re, im);
+                        po_term<R>(J, P, re, im);
                     }
                     if (CNT) {
                         cVisits += w.visits;
