@@ -407,7 +407,12 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
         int refs[W];
         for (int q = 0; q < W; ++q) {
             refs[q] = kEmptySlot;
-            if (q >= nk) continue;
+            if (q >= nk) {
+                // a box at +infinity on every axis: its slab distances are
+                // infinite, so no ray enters it
+                for (int k = 0; k < 6; ++k) r[W * k + q] = std::numeric_limits<float>::infinity();
+                continue;
+            }
             const RefNode<float>& c = b.nodes[kids[q]];
             for (int k = 0; k < 3; ++k) {
                 r[W * k + q] = c.lo[k];

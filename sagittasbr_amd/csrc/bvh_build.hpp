@@ -65,7 +65,7 @@ template <typename R> bool to_gpu_layout(const Bvh<R>& b, GpuBvh<R>& out, std::s
 // kWide descendants, opening the child with the largest surface area first;
 // leaves are the binary tree's leaves (same references). Record of
 // kWideRecBytes: lo.x[W] lo.y[W] lo.z[W] hi.x[W] hi.y[W] hi.z[W] ref[W], pad;
-// an unused slot holds ref kEmptySlot.
+// an unused slot holds ref kEmptySlot and a box at +infinity on every axis.
 struct GpuBvh4 {
     std::vector<uint8_t> nodes;
     int rootRef = 0;

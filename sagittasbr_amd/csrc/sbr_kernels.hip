@@ -205,15 +205,15 @@ __device__ __forceinline__ NodeW load_nodew(const void* nodes, int i) {
 
 // Sorting network on (key, value) pairs, ascending keys.
 template <int W> __device__ __forceinline__ void sort_net(float* e, int* r) {
+    // compare-exchange as selects (a branch costs exec-mask updates and moves)
     auto cs = [&](int a, int b) {
-        if (e[b] < e[a]) {
-            const float te = e[a];
-            e[a] = e[b];
-            e[b] = te;
-            const int tr = r[a];
-            r[a] = r[b];
-            r[b] = tr;
-        }
+        const bool sw = e[b] < e[a];
+        const float ea = sw ? e[b] : e[a], eb = sw ? e[a] : e[b];
+        const int ra = sw ? r[b] : r[a], rb = sw ? r[a] : r[b];
+        e[a] = ea;
+        e[b] = eb;
+        r[a] = ra;
+        r[b] = rb;
     };
     if constexpr (W == 4) {
         cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
@@ -465,9 +465,11 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
 #pragma unroll
         for (int q = 0; q < W; ++q) {
             float ent;
+            // an unused slot's box lies at +infinity: its entry is +infinity
+            // (above the finite lim) or its slab interval is empty
             const bool h = slab_grown(n.f[q], n.f[W + q], n.f[2 * W + q], n.f[3 * W + q], n.f[4 * W + q],
                                       n.f[5 * W + q], pLo, pHi, inv, ent) &&
-                           ent <= lim && n.ref(q) != kEmptySlot;
+                           ent <= lim;
             e[q] = h ? ent : 3.0e38f;
             r[q] = n.ref(q);
         }
