@@ -1028,6 +1028,10 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 const int sx = (tx * ts + (tid % perSide) * 16) >> 4, sy = (ty * ts + (tid / perSide) * 16) >> 4;
                 if (sx < J.passX && sy < J.passY) parts[J.passBase + int64_t(sy) * J.passX + sx] = AccT::zero();
             }
+            // every wave has read sPrev and sNext of the previous tile before
+            // thread 0 overwrites them below (a traced tile has the barriers
+            // of its pass sums)
+            __syncthreads();
         } else if constexpr (POOL) {
             const int perSide = ts >> 4;
             const int nPass = perSide * perSide;
