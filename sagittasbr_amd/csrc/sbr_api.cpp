@@ -386,6 +386,7 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
     pl->tileBase.assign(pl->nJobs, 0);
     pl->nTiles.assign(pl->nJobs, 0);
     int64_t base = 0;
+    int64_t recRays = 0, recIds = 0;
     int64_t rays = 0, passes = 0;
     int maxDepth = 0;
     const int ts = pl->tileSide;
@@ -463,6 +464,10 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         J.passX = any ? (i1 - i0 + 16) / 16 : 0;
         J.passY = any ? (j1 - j0 + 16) / 16 : 0;
         J.passBase = passes;
+        J.recBase = recRays;
+        J.recIdBase = recIds;
+        recRays += int64_t(j.n_x) * j.n_y;
+        recIds += int64_t(j.n_x) * j.n_y * j.max_bounces;
         pl->tileBase[i] = base;
         pl->nTiles[i] = J.nTiles;
         base += J.nTiles;
@@ -470,6 +475,8 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         rays += int64_t(J.nTiles) * ts * ts;
         if (mixed) {
             jobs64[i].tileBase = J.tileBase;
+            jobs64[i].recBase = J.recBase;
+            jobs64[i].recIdBase = J.recIdBase;
             jobs64[i].nTiles = J.nTiles;
             jobs64[i].passX = J.passX;
             jobs64[i].passY = J.passY;
@@ -571,7 +578,13 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     // the timed region covers the trace launches only
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
     if (pl->totalTiles > 0) {
-        if (pl->method == SBR_BATCHED && !record) {
+        if (record) {
+            // every job's tiles in one grid (each block finds its job); job
+            // i's records follow those of jobs 0..i-1
+            a.tileOffset = 0;
+            HIP_TRY(&ctx->err,
+                    launch_trace<R>(a, mode, sd, true, count, false, pool, ww, int(pl->totalTiles), s));
+        } else if (pl->method == SBR_BATCHED) {
             int occ = trace_occupancy<R>(mode, sd, pool);
             int per = pl->tune.wg_per_cu > 0 ? pl->tune.wg_per_cu : std::max(occ, 1);
             int64_t grid = int64_t(ctx->cuCount) * per;
@@ -1243,47 +1256,59 @@ int sbr_adjoint(sbr_ctx* ctx, int mesh_id, const sbr_job* jobs, int n_jobs, cons
     HIP_TRY(&ctx->err, dF.ensure(std::max(n_jobs, 1) * sizeof(double)));
     HIP_TRY(&ctx->err, hipMemsetAsync(dV.p, 0, nv * 3 * sizeof(double), ctx->stream));
     HIP_TRY(&ctx->err, hipMemsetAsync(dF.p, 0, std::max(n_jobs, 1) * sizeof(double), ctx->stream));
-    for (int i = 0; i < n_jobs; ++i) {
-        const sbr_job& jb = jobs[i];
-        // forward: mixed precision (FP64 outcome per ray), recording hit faces
+    if (n_jobs > 0) {
+        // forward of every direction in one record launch: mixed precision
+        // (FP64 outcome per ray), hit faces recorded, job i's records after
+        // job i-1's; then the adjoint launches, queued without host waits
         sbr_plan* pl = nullptr;
-        if (create_plan(ctx, &jb, 1, SBR_MIXED, SBR_PER_ANGLE, true, &pl)) return 1;
+        if (create_plan(ctx, jobs, n_jobs, SBR_MIXED, SBR_PER_ANGLE, true, &pl)) return 1;
         std::unique_ptr<sbr_plan, int (*)(sbr_plan*)> guard(pl, sbr_plan_destroy);
-        const size_t nr = size_t(jb.n_x) * jb.n_y;
-        const size_t mb = std::max(jb.max_bounces, 1);
-        HIP_TRY(&ctx->err, dIds.ensure(nr * mb * sizeof(int32_t)));
-        HIP_TRY(&ctx->err, dCnt.ensure(nr * sizeof(int32_t)));
-        HIP_TRY(&ctx->err, dL.ensure(nr * sizeof(double)));
-        HIP_TRY(&ctx->err, dRe.ensure(nr * 2 * sizeof(double)));
-        HIP_TRY(&ctx->err, pl->dRecFlag.ensure(nr));
-        HIP_TRY(&ctx->err, hipMemsetAsync(dIds.p, 0xff, nr * mb * sizeof(int32_t), ctx->stream));
-        HIP_TRY(&ctx->err, hipMemsetAsync(dCnt.p, 0, nr * sizeof(int32_t), ctx->stream));
+        size_t nrTot = 0, idTot = 0;
+        for (int i = 0; i < n_jobs; ++i) {
+            const size_t nr = size_t(jobs[i].n_x) * jobs[i].n_y;
+            nrTot += nr;
+            idTot += nr * size_t(jobs[i].max_bounces);
+        }
+        HIP_TRY(&ctx->err, dIds.ensure(std::max<size_t>(idTot, 1) * sizeof(int32_t)));
+        HIP_TRY(&ctx->err, dCnt.ensure(std::max<size_t>(nrTot, 1) * sizeof(int32_t)));
+        HIP_TRY(&ctx->err, dL.ensure(std::max<size_t>(nrTot, 1) * sizeof(double)));
+        HIP_TRY(&ctx->err, dRe.ensure(std::max<size_t>(nrTot, 1) * 2 * sizeof(double)));
+        HIP_TRY(&ctx->err, pl->dRecFlag.ensure(std::max<size_t>(nrTot, 1)));
+        HIP_TRY(&ctx->err, hipMemsetAsync(dIds.p, 0xff, idTot * sizeof(int32_t), ctx->stream));
+        HIP_TRY(&ctx->err, hipMemsetAsync(dCnt.p, 0, nrTot * sizeof(int32_t), ctx->stream));
         if (dispatch_launch(pl, true, false, static_cast<int32_t*>(dIds.p), static_cast<int32_t*>(dCnt.p),
                             static_cast<double*>(dL.p), static_cast<double*>(dRe.p)))
             return 1;
-        double reim[2], sg;
-        if (sbr_plan_results(pl, reim, &sg, nullptr)) return 1;
-        if (sigma) sigma[i] = sg;
-        // adjoint of this job's sigma along the recorded paths
-        AdjArgs a{};
-        make_frame<double>(jb.theta_deg, jb.phi_deg, jb.grid_size, a.D.llc, a.D.uv, a.D.vv, a.D.dir);
-        make_phys<double>(jb, a.D);
-        a.D.nx = jb.n_x;
-        a.D.ny = jb.n_y;
-        a.D.maxB = int(mb);
-        a.verts = static_cast<const double*>(m.adjVerts.p);
-        a.idx = static_cast<const unsigned*>(m.adjIdx.p);
-        a.ids = static_cast<const int32_t*>(dIds.p);
-        a.cnt = static_cast<const int32_t*>(dCnt.p);
-        a.Sre = reim[0];
-        a.Sim = reim[1];
-        a.w = dloss_dsigma ? dloss_dsigma[i] : 0.0;
-        a.freq = jb.freq;
-        a.scale = m.scale;
-        a.dvert = static_cast<double*>(dV.p);
-        a.dfreq = static_cast<double*>(dF.p) + i;
-        a.nRays = int(nr);
-        HIP_TRY(&ctx->err, launch_adjoint(a, ctx->stream));
+        std::vector<double> reim(size_t(2) * n_jobs), sg(n_jobs);
+        if (sbr_plan_results(pl, reim.data(), sg.data(), nullptr)) return 1;
+        size_t rayOff = 0, idOff = 0;
+        for (int i = 0; i < n_jobs; ++i) {
+            const sbr_job& jb = jobs[i];
+            if (sigma) sigma[i] = sg[i];
+            const size_t nr = size_t(jb.n_x) * jb.n_y;
+            // adjoint of this job's sigma along the recorded paths
+            AdjArgs a{};
+            make_frame<double>(jb.theta_deg, jb.phi_deg, jb.grid_size, a.D.llc, a.D.uv, a.D.vv, a.D.dir);
+            make_phys<double>(jb, a.D);
+            a.D.nx = jb.n_x;
+            a.D.ny = jb.n_y;
+            a.D.maxB = jb.max_bounces;
+            a.verts = static_cast<const double*>(m.adjVerts.p);
+            a.idx = static_cast<const unsigned*>(m.adjIdx.p);
+            a.ids = static_cast<const int32_t*>(dIds.p) + idOff;
+            a.cnt = static_cast<const int32_t*>(dCnt.p) + rayOff;
+            a.Sre = reim[2 * i];
+            a.Sim = reim[2 * i + 1];
+            a.w = dloss_dsigma ? dloss_dsigma[i] : 0.0;
+            a.freq = jb.freq;
+            a.scale = m.scale;
+            a.dvert = static_cast<double*>(dV.p);
+            a.dfreq = static_cast<double*>(dF.p) + i;
+            a.nRays = int(nr);
+            if (nr > 0) HIP_TRY(&ctx->err, launch_adjoint(a, ctx->stream));
+            rayOff += nr;
+            idOff += nr * size_t(jb.max_bounces);
+        }
         HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
     }
     if (dloss_dxyz && nv)

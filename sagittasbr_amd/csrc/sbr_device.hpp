@@ -35,6 +35,9 @@ template <typename R> struct alignas(16) Job {
     // (ray (i, j) sits at a = (i + 0.5) / nx), b likewise with cullB
     float cullA[4], cullB[4];
     int64_t tileBase;                // first global tile of this job
+    // record runs: this job's first ray in the per-ray arrays and first
+    // entry of the hit-face array (jobs one after another)
+    int64_t recBase, recIdBase;
     int nTiles;
     // Partial sums are kept per 16x16 ray pass on a grid fixed by the culled
     // rectangle alone (passX x passY), so the summation order -- and sigma to

@@ -1126,7 +1126,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     if (i >= J.nx || j >= J.ny) break;
                     if (CNT) cRays++;
                     const int ray = j * J.nx + i;
-                    int32_t* ids = REC ? a.recIds + size_t(ray) * J.maxB : nullptr;
+                    int32_t* ids = REC ? a.recIds + J.recIdBase + size_t(ray) * J.maxB : nullptr;
                     Work w{0, 0};
                     R re = 0, im = 0;
                     double dre = 0.0, dimg = 0.0;
@@ -1150,15 +1150,16 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     }
                     if constexpr (MODE == 1) {
                         if (CNT && fell) cRetrace++;
-                        if (REC) a.recFlag[ray] = fell ? 1 : 0;
+                        if (REC) a.recFlag[J.recBase + ray] = fell ? 1 : 0;
                     }
                     if (CNT) cBounces += P.count;
                     if (REC) {
                         for (int k = P.count; k < J.maxB; ++k) ids[k] = -1;
-                        a.recCount[ray] = P.count;
-                        a.recL[ray] = MODE == 1 ? P.Ld : double(P.L);
-                        a.recReIm[2 * size_t(ray)] = MODE == 1 ? dre : double(re);
-                        a.recReIm[2 * size_t(ray) + 1] = MODE == 1 ? dimg : double(im);
+                        const int64_t rr = J.recBase + ray;
+                        a.recCount[rr] = P.count;
+                        a.recL[rr] = MODE == 1 ? P.Ld : double(P.L);
+                        a.recReIm[2 * rr] = MODE == 1 ? dre : double(re);
+                        a.recReIm[2 * rr + 1] = MODE == 1 ? dimg : double(im);
                     }
                     if constexpr (MODE == 1) acc.add_ray64(dre, dimg);
                     else acc.add_ray(re, im);
