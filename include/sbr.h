@@ -20,10 +20,12 @@
  *     switch, include/RT/precision.h:29-101):
  *       SBR_FP32   FP32 traversal and physics (the reference's float build)
  *       SBR_FP64   FP64 throughout (the reference's USE_DOUBLE build)
- *       SBR_MIXED  FP32 traversal with error-bounded decisions; rays with an
- *                  ambiguous decision are re-traced in FP64, so every ray has
- *                  the FP64 outcome; FP64 path length and phase, FP32
- *                  two-sum compensated complex sum
+ *       SBR_MIXED  FP32 box traversal (boxes grown by the FP32 error
+ *                  bound) with FP64 triangle tests; a query the mixed
+ *                  query cannot vouch for (a near-tie, a hit on the edge of
+ *                  its box) is answered by the FP64 query, so every ray has
+ *                  the FP64 outcome; FP64 ray state, path length and phase,
+ *                  FP32 two-sum compensated complex sum
  */
 #ifndef SBR_H
 #define SBR_H

@@ -79,7 +79,7 @@ template <typename R> struct TraceArgs {
     void* partials;          // one accumulator per 16x16 ray pass
     double* jobOut;          // 2 doubles per job
     unsigned* jobDone;       // tiles finished per job
-    unsigned long long* tileCounter;  // persistent queue head
+    unsigned long long* tileCounter;  // persistent launch: next tile to take
     // per-ray records (single-job debug launches)
     int32_t* recIds;
     int32_t* recCount;

@@ -979,7 +979,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     // wave -> 8x8 quadrant of a 16x16 ray pass, lane -> ray within the quadrant
     const int qx = (wave & 1) * 8 + (lane & 7);
     const int qy = (wave >> 1) * 8 + (lane >> 3);
-    unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0, cRetrace = 0, cFull = 0;
+    unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0, cUndecided = 0, cFull = 0;
     typename Ov<R>::T ov[(MODE == 1 ? kMaxStack4 : kMaxStack) - SD];
     // mixed mode: private stack of the FP64 queries that answer ambiguous ones
     Ov<double>::T ov64[MODE == 1 ? kMaxStack : 1];
@@ -1082,7 +1082,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                             if constexpr (MODE == 1) {
                                 st.P.Ld = st.P.L;
                                 po_term<double, true>(a.jobs64[jid], st.P, re, im);
-                                if (CNT && fell) cRetrace++;
+                                if (CNT && fell) cUndecided++;
                                 fell = false;
                             } else {
                                 po_term<R>(J, st.P, re, im);
@@ -1149,7 +1149,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                         cTests += w.tests;
                     }
                     if constexpr (MODE == 1) {
-                        if (CNT && fell) cRetrace++;
+                        if (CNT && fell) cUndecided++;
                         if (REC) a.recFlag[J.recBase + ray] = fell ? 1 : 0;
                     }
                     if (CNT) cBounces += P.count;
@@ -1194,7 +1194,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     }
 
     if (CNT) {
-        unsigned long long c[7] = {cRays, cVisits, cTests, cBounces, cQueries, cRetrace, cFull};
+        unsigned long long c[7] = {cRays, cVisits, cTests, cBounces, cQueries, cUndecided, cFull};
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
             unsigned long long v = wave_sum(c[q]);
