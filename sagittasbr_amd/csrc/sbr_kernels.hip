@@ -969,6 +969,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     __shared__ int sRef[SD * kWgSize];
     __shared__ R sE[SD * kWgSize];
     __shared__ Acc<R, MODE> red[kWgSize / 64];
+    // wavefront sums of each 16x16 pass of a tile (at most 4 x 4 passes)
+    __shared__ Acc<R, MODE> sPass[16 * (kWgSize / 64)];
     __shared__ ST sRes[POOL ? 2 * kPoolRays : 1];
     __shared__ int64_t sNext;
     __shared__ unsigned sPrev;
@@ -1029,8 +1031,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 if (sx < J.passX && sy < J.passY) parts[J.passBase + int64_t(sy) * J.passX + sx] = AccT::zero();
             }
             // every wave has read sPrev and sNext of the previous tile before
-            // thread 0 overwrites them below (a traced tile has the barriers
-            // of its pass sums)
+            // thread 0 overwrites them below (a traced tile has the barrier
+            // before its pass sums)
             __syncthreads();
         } else if constexpr (POOL) {
             const int perSide = ts >> 4;
@@ -1114,9 +1116,13 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 }
             }
         } else
-        // a tile of side ts is covered by (ts/16)^2 passes of 16x16 rays; each
-        // pass is reduced on its own and stored at its place in the job's
-        // pass grid
+        // a tile of side ts is covered by (ts/16)^2 passes of 16x16 rays.
+        // Each wavefront sums its 8x8 patch of every pass without waiting for
+        // the others; after the tile the four patch sums of each pass are
+        // added in wavefront order (the order of wg_sum) and stored at the
+        // pass's place in the job's pass grid
+        {
+        const int perSide = ts >> 4;
         for (int py = 0; py < ts; py += 16) {
             for (int px = 0; px < ts; px += 16) {
                 AccT acc = AccT::zero();
@@ -1164,10 +1170,18 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     if constexpr (MODE == 1) acc.add_ray64(dre, dimg);
                     else acc.add_ray(re, im);
                 } while (false);
-                const AccT psum = wg_sum(acc, red, tid);
-                const int sx = (tx * ts + px) >> 4, sy = (ty * ts + py) >> 4;
-                if (tid == 0 && sx < J.passX && sy < J.passY) parts[J.passBase + int64_t(sy) * J.passX + sx] = psum;
+                acc.wave();
+                if (lane == 0) sPass[((py >> 4) * perSide + (px >> 4)) * (kWgSize / 64) + wave] = acc;
             }
+        }
+        __syncthreads();
+        if (tid < perSide * perSide) {
+            AccT r = sPass[tid * (kWgSize / 64)];
+#pragma unroll
+            for (int k = 1; k < kWgSize / 64; ++k) r.add(sPass[tid * (kWgSize / 64) + k]);
+            const int sx = (tx * ts + (tid % perSide) * 16) >> 4, sy = (ty * ts + (tid / perSide) * 16) >> 4;
+            if (sx < J.passX && sy < J.passY) parts[J.passBase + int64_t(sy) * J.passX + sx] = r;
+        }
         }
 
         if (tid == 0) {
