@@ -1184,8 +1184,13 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
         }
         }
 
+        // Release at device scope before the job's done-counter bump (the
+        // tile's partials are stored by lanes 0..15 of wavefront 0, before
+        // this in program order), acquire only in the workgroup that finishes
+        // the job: a full fence here would also invalidate the XCD's L2 --
+        // the BVH and triangles of every workgroup on it -- once per tile.
         if (tid == 0) {
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             sPrev = atomicAdd(&a.jobDone[jid], 1u);
             if (PERSIST) sNext = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
         }
@@ -1194,7 +1199,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
         const int64_t next = PERSIST ? uniform64(sNext) : a.totalTiles;
         if (prev == static_cast<unsigned>(J.nTiles - 1)) {
             // last tile of this job: fixed-order reduction of its partials
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             AccT s = AccT::zero();
             const int nPass = J.passX * J.passY;
             for (int q = tid; q < nPass; q += kWgSize) s.add(parts[J.passBase + q]);
