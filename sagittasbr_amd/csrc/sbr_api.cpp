@@ -28,6 +28,9 @@ template <typename R>
 hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool, bool ww,
                         int grid, hipStream_t s);
 hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s);
+template <typename R>
+hipError_t launch_finalize(const Job<R>* jobs, int nJobs, int mode, const void* partials, double* jobOut,
+                           hipStream_t s);
 size_t partial_bytes(bool fp64, int mode);
 hipError_t launch_adjoint(const AdjArgs& a, hipStream_t s);
 template <typename R> int trace_occupancy(int mode, int sd, bool pool);
@@ -129,7 +132,7 @@ struct sbr_plan {
     int maxDepth = 0;
     sbr_tuning tune{0, 0, 0, 0, 0};
     int tileSide = 32;
-    DevBuf dJobs, dPartials, dJobOut, dJobDone, dCounter, dCounters;
+    DevBuf dJobs, dPartials, dJobOut, dCounter, dCounters;
     // mixed mode
     DevBuf dJobs64, dRecFlag;
     std::vector<double> out;
@@ -493,7 +496,6 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
     HIP_TRY(&ctx->err, hipMemcpy(pl->dJobs.p, jobs.data(), sizeof(Job<R>) * pl->nJobs, hipMemcpyHostToDevice));
     HIP_TRY(&ctx->err, pl->dPartials.ensure(partial_bytes(sizeof(R) == 8, mode) * std::max<int64_t>(passes, 1)));
     HIP_TRY(&ctx->err, pl->dJobOut.ensure(sizeof(double) * 2 * std::max(pl->nJobs, 1)));
-    HIP_TRY(&ctx->err, pl->dJobDone.ensure(sizeof(unsigned) * std::max(pl->nJobs, 1)));
     HIP_TRY(&ctx->err, pl->dCounter.ensure(sizeof(unsigned long long)));
     HIP_TRY(&ctx->err, pl->dCounters.ensure(sizeof(unsigned long long) * 16));
     if (mixed) {
@@ -530,7 +532,7 @@ void release_plan_device(sbr_plan* pl) {
     if (pl->ev0) (void)hipEventDestroy(pl->ev0);
     if (pl->ev1) (void)hipEventDestroy(pl->ev1);
     pl->ev0 = pl->ev1 = nullptr;
-    for (DevBuf* b : {&pl->dJobs, &pl->dPartials, &pl->dJobOut, &pl->dJobDone, &pl->dCounter, &pl->dCounters,
+    for (DevBuf* b : {&pl->dJobs, &pl->dPartials, &pl->dJobOut, &pl->dCounter, &pl->dCounters,
                       &pl->dJobs64, &pl->dRecFlag})
         b->release();
     pl->ctx = nullptr;
@@ -556,7 +558,6 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     a.totalTiles = pl->totalTiles;
     a.partials = pl->dPartials.p;
     a.jobOut = static_cast<double*>(pl->dJobOut.p);
-    a.jobDone = static_cast<unsigned*>(pl->dJobDone.p);
     a.tileCounter = static_cast<unsigned long long*>(pl->dCounter.p);
     a.recIds = rIds;
     a.recCount = rCnt;
@@ -571,8 +572,6 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
         a.recFlag = static_cast<int8_t*>(pl->dRecFlag.p);
     }
     hipStream_t s = ctx->stream;
-    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobOut.p), size_t(4) * pl->nJobs, s));
-    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dJobDone.p), size_t(pl->nJobs), s));
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounter.p), 2, s));
     HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounters.p), 32, s));
     // the timed region covers the trace launches only
@@ -601,6 +600,8 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
         }
     }
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
+    // job sums from the pass partials, in each job's pass-grid order
+    HIP_TRY(&ctx->err, launch_finalize<R>(a.jobs, pl->nJobs, mode, pl->dPartials.p, a.jobOut, s));
     return 0;
 }
 

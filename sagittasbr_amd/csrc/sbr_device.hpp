@@ -78,7 +78,6 @@ template <typename R> struct TraceArgs {
     int64_t tileOffset;      // first tile of a non-persistent launch
     void* partials;          // one accumulator per 16x16 ray pass
     double* jobOut;          // 2 doubles per job
-    unsigned* jobDone;       // tiles finished per job
     unsigned long long* tileCounter;  // persistent launch: next tile to take
     // per-ray records (single-job debug launches)
     int32_t* recIds;

@@ -973,7 +973,6 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     __shared__ Acc<R, MODE> sPass[16 * (kWgSize / 64)];
     __shared__ ST sRes[POOL ? 2 * kPoolRays : 1];
     __shared__ int64_t sNext;
-    __shared__ unsigned sPrev;
     __shared__ unsigned sPool;
 
     const int tid = threadIdx.x;
@@ -1030,9 +1029,9 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 const int sx = (tx * ts + (tid % perSide) * 16) >> 4, sy = (ty * ts + (tid / perSide) * 16) >> 4;
                 if (sx < J.passX && sy < J.passY) parts[J.passBase + int64_t(sy) * J.passX + sx] = AccT::zero();
             }
-            // every wave has read sPrev and sNext of the previous tile before
-            // thread 0 overwrites them below (a traced tile has the barrier
-            // before its pass sums)
+            // every wave has read sNext of the previous tile before thread 0
+            // overwrites it below (a traced tile has the barrier before its
+            // pass sums)
             __syncthreads();
         } else if constexpr (POOL) {
             const int perSide = ts >> 4;
@@ -1184,30 +1183,15 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
         }
         }
 
-        // Release at device scope before the job's done-counter bump (the
-        // tile's partials are stored by lanes 0..15 of wavefront 0, before
-        // this in program order), acquire only in the workgroup that finishes
-        // the job: a full fence here would also invalidate the XCD's L2 --
-        // the BVH and triangles of every workgroup on it -- once per tile.
-        if (tid == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            sPrev = atomicAdd(&a.jobDone[jid], 1u);
-            if (PERSIST) sNext = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
-        }
-        __syncthreads();
-        const unsigned prev = uniform32(sPrev);
-        const int64_t next = PERSIST ? uniform64(sNext) : a.totalTiles;
-        if (prev == static_cast<unsigned>(J.nTiles - 1)) {
-            // last tile of this job: fixed-order reduction of its partials
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            AccT s = AccT::zero();
-            const int nPass = J.passX * J.passY;
-            for (int q = tid; q < nPass; q += kWgSize) s.add(parts[J.passBase + q]);
-            AccT tot = wg_sum(s, red, tid);
-            if (tid == 0) {
-                a.jobOut[2 * jid] = tot.dre();
-                a.jobOut[2 * jid + 1] = tot.dim();
-            }
+        // the next tile (persistent), broadcast through LDS; the job sums
+        // are formed after the launch (finalize_kernel), so a tile ends
+        // without fences: a device-scope release per tile writes the XCD's
+        // dirty L2 lines (the traversal stacks' scratch) back every time
+        int64_t next = a.totalTiles;
+        if constexpr (PERSIST) {
+            if (tid == 0) sNext = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
+            __syncthreads();
+            next = uniform64(sNext);
         }
         tile = next;
     }
@@ -1218,6 +1202,28 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
         for (int q = 0; q < 7; ++q) {
             unsigned long long v = wave_sum(c[q]);
             if (lane == 0) atomicAdd(&a.counters[q], v);
+        }
+    }
+}
+
+// Per-job sums of the pass partials in grid order, one workgroup per job
+// (grid-strided), after the trace launches of a run: the order is fixed by
+// the job's pass grid alone, so sigma does not depend on the launch shape.
+template <typename R, int MODE>
+__global__ __launch_bounds__(kWgSize) void finalize_kernel(const Job<R>* jobs, int nJobs, const void* partials,
+                                                          double* jobOut) {
+    using AccT = Acc<R, MODE>;
+    __shared__ AccT red[kWgSize / 64];
+    const AccT* parts = static_cast<const AccT*>(partials);
+    for (int jid = blockIdx.x; jid < nJobs; jid += gridDim.x) {
+        const Job<R>& J = jobs[jid];
+        AccT acc = AccT::zero();
+        const int nPass = J.passX * J.passY;
+        for (int q = threadIdx.x; q < nPass; q += kWgSize) acc.add(parts[J.passBase + q]);
+        const AccT tot = wg_sum(acc, red, static_cast<int>(threadIdx.x));
+        if (threadIdx.x == 0) {
+            jobOut[2 * jid] = tot.dre();
+            jobOut[2 * jid + 1] = tot.dim();
         }
     }
 }
@@ -1429,6 +1435,23 @@ template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool,
                                         hipStream_t);
 template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, bool, bool, int,
                                          hipStream_t);
+
+template <typename R>
+hipError_t launch_finalize(const Job<R>* jobs, int nJobs, int mode, const void* partials, double* jobOut,
+                           hipStream_t s) {
+    if (nJobs <= 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>(nJobs < 65535 ? nJobs : 65535));
+    if constexpr (std::is_same_v<R, float>) {
+        if (mode == 1) {
+            hipLaunchKernelGGL((finalize_kernel<float, 1>), grid, dim3(kWgSize), 0, s, jobs, nJobs, partials, jobOut);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((finalize_kernel<R, 0>), grid, dim3(kWgSize), 0, s, jobs, nJobs, partials, jobOut);
+    return hipGetLastError();
+}
+template hipError_t launch_finalize<float>(const Job<float>*, int, int, const void*, double*, hipStream_t);
+template hipError_t launch_finalize<double>(const Job<double>*, int, int, const void*, double*, hipStream_t);
 
 hipError_t launch_adjoint(const AdjArgs& a, hipStream_t s) {
     if (a.nRays <= 0) return hipSuccess;
