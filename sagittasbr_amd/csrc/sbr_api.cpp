@@ -87,7 +87,8 @@ template <typename R> struct PrecMesh {
     Bvh<R> bvh;
     GpuBvh<R> gpu;
     DevBuf nodes, tris;
-    DevBuf tris64;  // FP32 tree: FP64 vertex records in tree order (80 B each)
+    DevBuf tris64;  // FP32 tree: FP64 vertex records in tree order (80 B each), unless vertsExact
+    bool vertsExact = false;  // FP32 tree: every FP64 vertex is its FP32 vertex widened
     GpuBvh4 gpu4;   // FP32 tree: four-wide collapse (mixed mode)
     DevBuf nodes4;
     DevBuf cull;    // boxes of the per-tile emptiness test
@@ -148,6 +149,33 @@ template <typename R> PrecMesh<R>& prec_of(MeshEntry& m);
 template <> PrecMesh<float>& prec_of<float>(MeshEntry& m) { return m.f32; }
 template <> PrecMesh<double>& prec_of<double>(MeshEntry& m) { return m.f64; }
 
+// A model scale that rounds FP32 vertices leaves each FP32 vertex within 2
+// ulps of the reference FP64 one (the FP32 scale's rounding plus the
+// product's): the mixed mode's FP32 boxes are then widened by kWidenUlps
+// outwards so that they bound the FP64 triangles it tests.
+constexpr int kWidenUlps = 3;
+float widen_ulps(float v, int n) {
+    for (int k = 0; k < (n < 0 ? -n : n); ++k) v = std::nextafter(v, n < 0 ? -INFINITY : INFINITY);
+    return v;
+}
+void widen_boxes4(GpuBvh4& g) {
+    const size_t nRec = g.nodes.size() / kWideRecBytes;
+    for (size_t i = 0; i < nRec; ++i) {
+        float f[kWideRecBytes / 4];
+        std::memcpy(f, g.nodes.data() + i * kWideRecBytes, kWideRecBytes);
+        for (int q = 0; q < kWide; ++q) {
+            int ref;
+            std::memcpy(&ref, &f[6 * kWide + q], sizeof(int));
+            if (ref == kEmptySlot) continue;
+            for (int k = 0; k < 3; ++k) {
+                f[k * kWide + q] = widen_ulps(f[k * kWide + q], -kWidenUlps);
+                f[(3 + k) * kWide + q] = widen_ulps(f[(3 + k) * kWide + q], kWidenUlps);
+            }
+        }
+        std::memcpy(g.nodes.data() + i * kWideRecBytes, f, kWideRecBytes);
+    }
+}
+
 // Triangle soup in R (src/app/rtRcsEntry.cu:118-148): float position
 // converted to R, times the model scale in R.
 template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
@@ -172,32 +200,53 @@ template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
     HIP_TRY(&ctx->err, pm.tris.ensure(pm.gpu.tris.size()));
     HIP_TRY(&ctx->err, hipMemcpy(pm.nodes.p, pm.gpu.nodes.data(), pm.gpu.nodes.size(), hipMemcpyHostToDevice));
     HIP_TRY(&ctx->err, hipMemcpy(pm.tris.p, pm.gpu.tris.data(), pm.gpu.tris.size(), hipMemcpyHostToDevice));
-    {
-        const std::vector<float> cb = cull_boxes<R>(pm.bvh, kCullBoxes);
-        pm.nCull = int(cb.size() / 6);
-        HIP_TRY(&ctx->err, pm.cull.ensure(cb.size() * sizeof(float)));
-        HIP_TRY(&ctx->err, hipMemcpy(pm.cull.p, cb.data(), cb.size() * sizeof(float), hipMemcpyHostToDevice));
-    }
     if constexpr (std::is_same_v<R, float>) {
         // reference FP64 vertices (rtRcsEntry.cu:136-144 with Real = double)
         // of every triangle in this tree's order, for the mixed mode
         const double sd = m.scale;
         std::vector<double> t64(nt * 10);
+        bool exact = true;
         for (size_t h = 0; h < nt; ++h) {
             const size_t f = size_t(pm.bvh.perm[h]);
             for (int c = 0; c < 3; ++c) {
                 uint32_t vi = m.idx[3 * f + c];
-                for (int k = 0; k < 3; ++k) t64[10 * h + 3 * c + k] = double(m.xyz[3 * size_t(vi) + k]) * sd;
+                for (int k = 0; k < 3; ++k) {
+                    const float x = m.xyz[3 * size_t(vi) + k];
+                    t64[10 * h + 3 * c + k] = double(x) * sd;
+                    // the FP32 soup's value of the same vertex
+                    exact = exact && double(x * s) == t64[10 * h + 3 * c + k];
+                }
             }
             long long fid = static_cast<long long>(f);
             std::memcpy(&t64[10 * h + 9], &fid, sizeof(double));
         }
+        pm.vertsExact = exact;
         if (!to_gpu_layout4(pm.bvh, pm.gpu4, err)) return fail(&ctx->err, err);
+        if (!exact) widen_boxes4(pm.gpu4);
         HIP_TRY(&ctx->err, pm.nodes4.ensure(pm.gpu4.nodes.size()));
         HIP_TRY(&ctx->err,
                 hipMemcpy(pm.nodes4.p, pm.gpu4.nodes.data(), pm.gpu4.nodes.size(), hipMemcpyHostToDevice));
-        HIP_TRY(&ctx->err, pm.tris64.ensure(t64.size() * sizeof(double)));
-        HIP_TRY(&ctx->err, hipMemcpy(pm.tris64.p, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (!exact) {
+            HIP_TRY(&ctx->err, pm.tris64.ensure(t64.size() * sizeof(double)));
+            HIP_TRY(&ctx->err,
+                    hipMemcpy(pm.tris64.p, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    {
+        std::vector<float> cb = cull_boxes<R>(pm.bvh, kCullBoxes);
+        // (FP32 tree whose vertices round: the boxes must bound the FP64
+        // triangles of the mixed mode too)
+        if constexpr (std::is_same_v<R, float>) {
+            if (!pm.vertsExact)
+                for (size_t b = 0; b < cb.size() / 6; ++b)
+                    for (int k = 0; k < 3; ++k) {
+                        cb[6 * b + k] = widen_ulps(cb[6 * b + k], -kWidenUlps);
+                        cb[6 * b + 3 + k] = widen_ulps(cb[6 * b + 3 + k], kWidenUlps);
+                    }
+        }
+        pm.nCull = int(cb.size() / 6);
+        HIP_TRY(&ctx->err, pm.cull.ensure(cb.size() * sizeof(float)));
+        HIP_TRY(&ctx->err, hipMemcpy(pm.cull.p, cb.data(), cb.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     pm.built = true;
     pm.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -227,6 +276,11 @@ template <typename R> int upload_table(sbr_ctx* ctx) {
             if constexpr (std::is_same_v<R, float>) {
                 mm.nodes4 = pm.nodes4.p;
                 mm.rootRef4 = pm.gpu4.rootRef;
+                mm.vertsExact = pm.vertsExact ? 1 : 0;
+                for (int k = 0; k < 3; ++k) {
+                    mm.rootLo4[k] = pm.vertsExact ? pm.gpu.rootLo[k] : widen_ulps(pm.gpu.rootLo[k], -kWidenUlps);
+                    mm.rootHi4[k] = pm.vertsExact ? pm.gpu.rootHi[k] : widen_ulps(pm.gpu.rootHi[k], kWidenUlps);
+                }
             }
             for (int k = 0; k < 3; ++k) {
                 mm.rootLo[k] = pm.gpu.rootLo[k];

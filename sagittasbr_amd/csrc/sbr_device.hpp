@@ -51,8 +51,12 @@ template <typename R> struct alignas(16) Mesh {
     const void* tris;
     // FP32 tree only: the reference FP64 vertices of each triangle, in the
     // FP32 tree's order (v0 v1 v2 as 9 doubles + face id), for the mixed
-    // mode's FP64 triangle tests
+    // mode's FP64 triangle tests; null when vertsExact
     const void* tris64;
+    // FP32 tree only: every reference FP64 vertex equals its FP32 vertex
+    // (model scale 1, or any scale that rounds no vertex), so the mixed
+    // mode reads the FP32 record and widens it exactly
+    int vertsExact;
     // FP32 tree only: the four-wide collapse of the tree (mixed mode)
     const void* nodes4;
     // up to kCullBoxes boxes (lo.xyz hi.xyz, FP32, rounded outwards) that
@@ -60,6 +64,9 @@ template <typename R> struct alignas(16) Mesh {
     const float* cull;
     int nCull;
     R rootLo[3], rootHi[3];
+    // FP32 tree only: the root box for the mixed mode, bounding the FP64
+    // triangles (the FP32 root box widened when vertsExact is 0)
+    float rootLo4[3], rootHi4[3];
     // 1 + the largest coordinate magnitude of the root box per axis, summed
     // (mixed mode's tie margin scale)
     double tieScale;

@@ -151,6 +151,22 @@ __device__ __forceinline__ void load_tri_raw64(const void* tris64, int i, V3<dou
     v2 = {d.x, d.y, e.x};
     face = static_cast<int>(__double_as_longlong(e.y));
 }
+// Reference FP64 vertices and face id of FP32-tree triangle i for the mixed
+// mode: the FP32 record widened when it holds them exactly, else the FP64
+// record.
+__device__ __forceinline__ void load_tri_mixed(const Mesh<float>& m, int i, V3<double>& v0, V3<double>& v1,
+                                               V3<double>& v2, int& face) {
+    if (m.vertsExact) {
+        const gptr<vf4> p = as_global<vf4>(m.tris) + 3 * i;
+        const vf4 a = p[0], b = p[1], c = p[2];
+        v0 = {double(a.x), double(a.y), double(a.z)};
+        v1 = {double(a.w), double(b.x), double(b.y)};
+        v2 = {double(b.z), double(b.w), double(c.x)};
+        face = __float_as_int(c.y);
+    } else {
+        load_tri_raw64(m.tris64, i, v0, v1, v2, face);
+    }
+}
 __device__ __forceinline__ TriRec<double> load_tri(const double* tris, int i) {
     gptr<vd2> p = as_global<vd2>(tris) + 5 * i;
     vd2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
@@ -424,9 +440,9 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
                                                  int& hFace) {
     const float o1 = fabsf(o.x) + fabsf(o.y) + fabsf(o.z);
     // far distance of anything in the root box from the origin
-    float cx = 0.5f * (m.rootLo[0] + m.rootHi[0]), cy = 0.5f * (m.rootLo[1] + m.rootHi[1]),
-          cz = 0.5f * (m.rootLo[2] + m.rootHi[2]);
-    float hx = m.rootHi[0] - cx, hy = m.rootHi[1] - cy, hz = m.rootHi[2] - cz;
+    float cx = 0.5f * (m.rootLo4[0] + m.rootHi4[0]), cy = 0.5f * (m.rootLo4[1] + m.rootHi4[1]),
+          cz = 0.5f * (m.rootLo4[2] + m.rootHi4[2]);
+    float hx = m.rootHi4[0] - cx, hy = m.rootHi4[1] - cy, hz = m.rootHi4[2] - cz;
     // (L1 norms bound the Euclidean ones)
     float tFar = (fabsf(o.x - cx) + fabsf(o.y - cy) + fabsf(o.z - cz) + hx + hy + hz) * (1.0f + 4.0f * kEps32);
     // position and distance error bounds of the FP32 box tests over the query
@@ -442,7 +458,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     const float dxs = dx * (1.0f + 8.0f * kEps32) + 4.0f * kEps32 * o1;
     const V3<float> pLo{(o.x + dxs) * inv.x, (o.y + dxs) * inv.y, (o.z + dxs) * inv.z};
     const V3<float> pHi{(o.x - dxs) * inv.x, (o.y - dxs) * inv.y, (o.z - dxs) * inv.z};
-    if (!slab_grown(m.rootLo[0], m.rootLo[1], m.rootLo[2], m.rootHi[0], m.rootHi[1], m.rootHi[2], pLo, pHi, inv,
+    if (!slab_grown(m.rootLo4[0], m.rootLo4[1], m.rootLo4[2], m.rootHi4[0], m.rootHi4[1], m.rootHi4[2], pLo, pHi, inv,
                     e)) {
         tHit = closest;
         return -1;
@@ -489,7 +505,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
         for (int k = 0; k < cnt; ++k) {
             V3<double> a, b, c;
             int f;
-            load_tri_raw64(m.tris64, first + k, a, b, c, f);
+            load_tri_mixed(m, first + k, a, b, c, f);
             const TriRec<double> tr{a, sub(b, a), sub(c, a), f};
             double t;
             if (mt_test(tr, o64, d64, 1e20, t)) {
@@ -533,7 +549,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
         if (second - closest <= margin) {
             amb = true;
         } else {
-            load_tri_raw64(m.tris64, hit, hv0, hv1, hv2, hFace);
+            load_tri_mixed(m, hit, hv0, hv1, hv2, hFace);
             auto inside_axis = [&](double pa, double x0, double x1, double x2) {
                 const double lo = fmin(x0, fmin(x1, x2)), hi = fmax(x0, fmax(x1, x2));
                 return !(hi > lo) || (pa - lo > margin && hi - pa > margin);

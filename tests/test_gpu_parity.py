@@ -127,6 +127,28 @@ def test_per_ray_mixed_has_fp64_outcome(ctx, sphere, tri_duct, case, angles):
         assert flagged > 0, "the FP64 answer path was not exercised"
 
 
+@pytest.mark.parametrize("scale", [0.37, 2.0])
+def test_per_ray_mixed_scaled_model(ctx, scale):
+    """A model scale that rounds the FP32 vertices (0.37) makes the mixed
+    mode read the reference FP64 vertex records; a power of two (2.0) keeps
+    every FP64 vertex equal to its FP32 one, so the FP32 record is widened.
+    Both must give the FP64 oracle's rays, built at the same scale."""
+    v, f = M.trihedral_duct_scene()
+    mesh = ctx.mesh(v, f, scale=scale)
+    s64 = O.Scene(v, f, 1, scale=scale)
+    kw = dict(C2S, grid_size=1.6 * scale)
+    p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    for a in (4140, 2790, 7188):
+        print(f"[per-ray mixed] tri_duct scale {scale} angle {a}", flush=True)
+        g = ctx.trace_rays(mesh, p, a, S.SBR_MIXED)
+        o = s64.trace_angle(op, a)
+        assert np.array_equal(g["count"], o["count"])
+        assert np.array_equal(g["ids"], o["ids"])
+        assert np.array_equal(g["L"], o["L"])
+        np.testing.assert_allclose(g["reim"], o["reim"], rtol=0, atol=1e-12 * np.abs(o["reim"]).max())
+    ctx.destroy_mesh(mesh)
+
+
 def test_sphere_sigma_all_precisions(ctx, sphere):
     v, f, mesh, s32, s64 = sphere
     kw = dict(C1, phi_end=350, phi_samples=36)
