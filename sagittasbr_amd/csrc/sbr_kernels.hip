@@ -5,9 +5,9 @@
 // integratePoMultiBounce, src/cuda/poKernels.cu:66-122):
 //   ray generation in registers -> closest-hit BVH traversal with
 //   Moller-Trumbore -> multi-bounce mirror loop -> PO term in registers ->
-//   wavefront shuffle reduction -> per-tile partial -> the last tile of each
-//   job reduces that job's partials in a fixed order (deterministic, no
-//   float atomics, no per-ray buffers in HBM).
+//   wavefront shuffle reduction -> one partial per 16x16 ray pass -> a
+//   finalize launch sums each job's partials in a fixed order
+//   (deterministic, no float atomics, no per-ray buffers in HBM).
 //
 // Work unit: a tile of tileSide x tileSide parallel rays of one job (one
 // sigma value). Each 256-thread workgroup is 4 wavefronts; a wavefront owns
@@ -1002,12 +1002,11 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     Ov<double>::T ov64[MODE == 1 ? kMaxStack : 1];
 
     // Persistent mode pops tiles from a global counter. The pop for the next
-    // tile is made by thread 0 in the same block that bumps the job's
-    // done-counter, so the loop head holds no thread-0-only code: a loop-head
-    // `if (tid == 0)` next to the loop-tail `if (tid == 0)` store lets the
-    // compiler thread the two into a nested loop whose inner back edge skips
-    // the pop for lanes 1..63 of wave 0 (seen in the generated code as a
-    // loop that never ends).
+    // tile is made by thread 0 at the loop tail only, so the loop head holds
+    // no thread-0-only code: a loop-head `if (tid == 0)` next to a loop-tail
+    // `if (tid == 0)` store lets the compiler thread the two into a nested
+    // loop whose inner back edge skips the pop for lanes 1..63 of wave 0
+    // (seen in the generated code as a loop that never ends).
     int64_t tile = a.tileOffset + blockIdx.x;
     // The pool counter is never reset: every lane ends each batch with
     // exactly one failed fetch, so a batch of n rays advances it by
