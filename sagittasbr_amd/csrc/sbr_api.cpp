@@ -32,7 +32,7 @@ template <typename R>
 hipError_t launch_finalize(const Job<R>* jobs, int nJobs, int mode, const void* partials, double* jobOut,
                            hipStream_t s);
 size_t partial_bytes(bool fp64, int mode);
-hipError_t launch_adjoint(const AdjArgs& a, hipStream_t s);
+hipError_t launch_adjoint(const AdjArgs& a, int blocks, hipStream_t s);
 template <typename R> int trace_occupancy(int mode, int sd, bool pool);
 }  // namespace sbr
 
@@ -1314,16 +1314,40 @@ int sbr_adjoint(sbr_ctx* ctx, int mesh_id, const sbr_job* jobs, int n_jobs, cons
     if (n_jobs > 0) {
         // forward of every direction in one record launch: mixed precision
         // (FP64 outcome per ray), hit faces recorded, job i's records after
-        // job i-1's; then the adjoint launches, queued without host waits
+        // job i-1's; then one adjoint launch over every job's rays, which
+        // reads the jobs' sums from the forward's output (no host wait)
         sbr_plan* pl = nullptr;
         if (create_plan(ctx, jobs, n_jobs, SBR_MIXED, SBR_PER_ANGLE, true, &pl)) return 1;
         std::unique_ptr<sbr_plan, int (*)(sbr_plan*)> guard(pl, sbr_plan_destroy);
+        std::vector<AdjJob> aj(n_jobs);
         size_t nrTot = 0, idTot = 0;
+        int blocks = 0;
         for (int i = 0; i < n_jobs; ++i) {
-            const size_t nr = size_t(jobs[i].n_x) * jobs[i].n_y;
+            const sbr_job& jb = jobs[i];
+            const size_t nr = size_t(jb.n_x) * jb.n_y;
+            if (nr > size_t(INT32_MAX) - 255) return fail(&ctx->err, "job " + std::to_string(i) + " has too many rays");
+            AdjJob& g = aj[i];
+            g = AdjJob{};
+            make_frame<double>(jb.theta_deg, jb.phi_deg, jb.grid_size, g.D.llc, g.D.uv, g.D.vv, g.D.dir);
+            make_phys<double>(jb, g.D);
+            g.D.nx = jb.n_x;
+            g.D.ny = jb.n_y;
+            g.D.maxB = jb.max_bounces;
+            g.rayOff = int64_t(nrTot);
+            g.idOff = int64_t(idTot);
+            g.w = dloss_dsigma ? dloss_dsigma[i] : 0.0;
+            g.freq = jb.freq;
+            g.nRays = int(nr);
+            g.blockBase = blocks;
+            const int64_t nb = int64_t(nr + 255) / 256;
+            if (int64_t(blocks) + nb > INT32_MAX) return fail(&ctx->err, "adjoint launch too large");
+            blocks += int(nb);
             nrTot += nr;
-            idTot += nr * size_t(jobs[i].max_bounces);
+            idTot += nr * size_t(jb.max_bounces);
         }
+        DevBuf dAdj;
+        HIP_TRY(&ctx->err, dAdj.ensure(sizeof(AdjJob) * aj.size()));
+        HIP_TRY(&ctx->err, hipMemcpy(dAdj.p, aj.data(), sizeof(AdjJob) * aj.size(), hipMemcpyHostToDevice));
         HIP_TRY(&ctx->err, dIds.ensure(std::max<size_t>(idTot, 1) * sizeof(int32_t)));
         HIP_TRY(&ctx->err, dCnt.ensure(std::max<size_t>(nrTot, 1) * sizeof(int32_t)));
         HIP_TRY(&ctx->err, dL.ensure(std::max<size_t>(nrTot, 1) * sizeof(double)));
@@ -1334,37 +1358,22 @@ int sbr_adjoint(sbr_ctx* ctx, int mesh_id, const sbr_job* jobs, int n_jobs, cons
         if (dispatch_launch(pl, true, false, static_cast<int32_t*>(dIds.p), static_cast<int32_t*>(dCnt.p),
                             static_cast<double*>(dL.p), static_cast<double*>(dRe.p)))
             return 1;
-        std::vector<double> reim(size_t(2) * n_jobs), sg(n_jobs);
-        if (sbr_plan_results(pl, reim.data(), sg.data(), nullptr)) return 1;
-        size_t rayOff = 0, idOff = 0;
-        for (int i = 0; i < n_jobs; ++i) {
-            const sbr_job& jb = jobs[i];
-            if (sigma) sigma[i] = sg[i];
-            const size_t nr = size_t(jb.n_x) * jb.n_y;
-            // adjoint of this job's sigma along the recorded paths
-            AdjArgs a{};
-            make_frame<double>(jb.theta_deg, jb.phi_deg, jb.grid_size, a.D.llc, a.D.uv, a.D.vv, a.D.dir);
-            make_phys<double>(jb, a.D);
-            a.D.nx = jb.n_x;
-            a.D.ny = jb.n_y;
-            a.D.maxB = jb.max_bounces;
-            a.verts = static_cast<const double*>(m.adjVerts.p);
-            a.idx = static_cast<const unsigned*>(m.adjIdx.p);
-            a.ids = static_cast<const int32_t*>(dIds.p) + idOff;
-            a.cnt = static_cast<const int32_t*>(dCnt.p) + rayOff;
-            a.Sre = reim[2 * i];
-            a.Sim = reim[2 * i + 1];
-            a.w = dloss_dsigma ? dloss_dsigma[i] : 0.0;
-            a.freq = jb.freq;
-            a.scale = m.scale;
-            a.dvert = static_cast<double*>(dV.p);
-            a.dfreq = static_cast<double*>(dF.p) + i;
-            a.nRays = int(nr);
-            if (nr > 0) HIP_TRY(&ctx->err, launch_adjoint(a, ctx->stream));
-            rayOff += nr;
-            idOff += nr * size_t(jb.max_bounces);
-        }
-        HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
+        AdjArgs a{};
+        a.jobs = static_cast<const AdjJob*>(dAdj.p);
+        a.nJobs = n_jobs;
+        a.verts = static_cast<const double*>(m.adjVerts.p);
+        a.idx = static_cast<const unsigned*>(m.adjIdx.p);
+        a.ids = static_cast<const int32_t*>(dIds.p);
+        a.cnt = static_cast<const int32_t*>(dCnt.p);
+        a.sums = static_cast<const double*>(pl->dJobOut.p);
+        a.scale = m.scale;
+        a.dvert = static_cast<double*>(dV.p);
+        a.dfreq = static_cast<double*>(dF.p);
+        HIP_TRY(&ctx->err, launch_adjoint(a, blocks, ctx->stream));
+        std::vector<double> sg(n_jobs);
+        if (sbr_plan_results(pl, nullptr, sg.data(), nullptr)) return 1;  // waits for the stream
+        if (sigma)
+            for (int i = 0; i < n_jobs; ++i) sigma[i] = sg[i];
     }
     if (dloss_dxyz && nv)
         HIP_TRY(&ctx->err, hipMemcpy(dloss_dxyz, dV.p, nv * 3 * sizeof(double), hipMemcpyDeviceToHost));

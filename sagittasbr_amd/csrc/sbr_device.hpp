@@ -104,18 +104,27 @@ template <typename R> struct TraceArgs {
 };
 
 // Arguments of the frozen-path adjoint kernel (one launch per job).
+// One job of an adjoint launch; its rays take blocks [blockBase, next
+// job's blockBase), 256 rays each.
+struct AdjJob {
+    Job<double> D;           // frame, physics, n_x, n_y, max bounces
+    int64_t rayOff, idOff;   // the job's first entry in the record arrays
+    double w;                // upstream d loss / d sigma of the job
+    double freq;
+    int nRays;
+    int blockBase;
+};
 struct AdjArgs {
-    Job<double> D;
+    const AdjJob* jobs;
+    int nJobs;
     const double* verts;  // n_verts * 3, reference FP64 vertices (scaled)
     const unsigned* idx;  // n_tris * 3
-    const int32_t* ids;   // rays * maxB original face ids
-    const int32_t* cnt;   // rays
-    double Sre, Sim;      // the job's complex sum
-    double w;             // upstream d loss / d sigma of the job
-    double freq, scale;
+    const int32_t* ids;   // hit faces of every job's rays (max bounces per ray)
+    const int32_t* cnt;   // bounce counts of every job's rays
+    const double* sums;   // 2 per job: the forward complex sums
+    double scale;
     double* dvert;        // n_verts * 3, d loss / d (input coordinate), accumulated
-    double* dfreq;        // d sigma / d f of the job (unweighted), accumulated
-    int nRays;
+    double* dfreq;        // per job: d sigma / d f (unweighted), accumulated
 };
 
 }  // namespace sbr

@@ -1300,12 +1300,20 @@ __device__ __forceinline__ void adj_add(double* p, V3<double> g, double s) {
 
 __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     __shared__ double red[4];
-    const int ray = blockIdx.x * 256 + threadIdx.x;
+    // the job of this block: the last one whose first block is <= blockIdx.x
+    int lo = 0, hi = a.nJobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.jobs[mid].blockBase <= static_cast<int>(blockIdx.x)) lo = mid;
+        else hi = mid - 1;
+    }
+    const AdjJob& G = a.jobs[lo];
+    const int ray = (static_cast<int>(blockIdx.x) - G.blockBase) * 256 + static_cast<int>(threadIdx.x);
     double dfr = 0.0;
-    const int B = ray < a.nRays ? a.cnt[ray] : 0;
+    const int B = ray < G.nRays ? a.cnt[G.rayOff + ray] : 0;
     if (B > 0) {
-        const Job<double>& D = a.D;
-        const int32_t* fids = a.ids + size_t(ray) * D.maxB;
+        const Job<double>& D = G.D;
+        const int32_t* fids = a.ids + G.idOff + size_t(ray) * D.maxB;
         const int i = ray % D.nx, j = ray / D.nx;
         const V3<double> o0 = ray_origin(D, i, j);
         const V3<double> d0{D.dir[0], D.dir[1], D.dir[2]};
@@ -1328,13 +1336,13 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
             sincos(ph, &sn, &cs);
             const double cre = -mag * sn, cim = mag * cs;
             // d sigma / d (Re c, Im c) = 8 pi (Re S, Im S)
-            const double gre = 8.0 * M_PI * a.Sre, gim = 8.0 * M_PI * a.Sim;
+            const double gre = 8.0 * M_PI * a.sums[2 * lo], gim = 8.0 * M_PI * a.sums[2 * lo + 1];
             const double rot = gre * (-cim) + gim * cre;  // g . (i c)
             const double par = gre * cre + gim * cim;     // g . c
-            dfr = (par + 2.0 * L * D.kd * rot) / a.freq;
-            const double gL = a.w * 2.0 * D.kd * rot;
-            const double gC = a.w * par / cosT;
-            if (a.w != 0.0) {
+            dfr = (par + 2.0 * L * D.kd * rot) / G.freq;
+            const double gL = G.w * 2.0 * D.kd * rot;
+            const double gC = G.w * par / cosT;
+            if (G.w != 0.0) {
                 V3<double> ob{0, 0, 0}, db{0, 0, 0};  // adjoints of o_{b+1}, d_{b+1}
                 for (int b = B - 1; b >= 0; --b) {
                     // o_b, d_b: forward again from the ray start
@@ -1396,7 +1404,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     dfr = wave_sum(dfr);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dfr;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(a.dfreq, red[0] + red[1] + red[2] + red[3]);
+    if (threadIdx.x == 0) atomicAdd(a.dfreq + lo, red[0] + red[1] + red[2] + red[3]);
 }
 
 template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW>
@@ -1468,9 +1476,9 @@ hipError_t launch_finalize(const Job<R>* jobs, int nJobs, int mode, const void* 
 template hipError_t launch_finalize<float>(const Job<float>*, int, int, const void*, double*, hipStream_t);
 template hipError_t launch_finalize<double>(const Job<double>*, int, int, const void*, double*, hipStream_t);
 
-hipError_t launch_adjoint(const AdjArgs& a, hipStream_t s) {
-    if (a.nRays <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adjoint_kernel, dim3((a.nRays + 255) / 256), dim3(256), 0, s, a);
+hipError_t launch_adjoint(const AdjArgs& a, int blocks, hipStream_t s) {
+    if (blocks <= 0 || a.nJobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adjoint_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
