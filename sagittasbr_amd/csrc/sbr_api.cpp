@@ -130,6 +130,8 @@ struct sbr_plan {
     std::vector<int> nTiles;
     std::vector<int> meshIds;  // distinct meshes of the jobs
     int64_t totalTiles = 0;
+    DevBuf dChunkJob;  // job of the first tile of each kChunkTiles chunk
+    int64_t nChunks = 0;
     int maxDepth = 0;
     sbr_tuning tune{0, 0, 0, 0, 0};
     int tileSide = 32;
@@ -542,6 +544,19 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         jobs[i] = J;
     }
     pl->totalTiles = base;
+    {
+        // tile -> job table: the last job whose first tile is <= chunk * kChunkTiles
+        pl->nChunks = (base + kChunkTiles - 1) / kChunkTiles;
+        std::vector<int> cj(std::max<int64_t>(pl->nChunks, 1), 0);
+        int jj = 0;
+        for (int64_t c = 0; c < pl->nChunks; ++c) {
+            const int64_t t = c * kChunkTiles;
+            while (jj + 1 < pl->nJobs && pl->tileBase[jj + 1] <= t) ++jj;
+            cj[size_t(c)] = jj;
+        }
+        HIP_TRY(&ctx->err, pl->dChunkJob.ensure(cj.size() * sizeof(int)));
+        HIP_TRY(&ctx->err, hipMemcpy(pl->dChunkJob.p, cj.data(), cj.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     pl->maxDepth = maxDepth;
     if (upload_table<R>(ctx)) return 1;
     if (mixed && upload_table<double>(ctx)) return 1;
@@ -586,7 +601,7 @@ void release_plan_device(sbr_plan* pl) {
     if (pl->ev0) (void)hipEventDestroy(pl->ev0);
     if (pl->ev1) (void)hipEventDestroy(pl->ev1);
     pl->ev0 = pl->ev1 = nullptr;
-    for (DevBuf* b : {&pl->dJobs, &pl->dPartials, &pl->dJobOut, &pl->dCounter, &pl->dCounters,
+    for (DevBuf* b : {&pl->dJobs, &pl->dPartials, &pl->dJobOut, &pl->dCounter, &pl->dCounters, &pl->dChunkJob,
                       &pl->dJobs64, &pl->dRecFlag})
         b->release();
     pl->ctx = nullptr;
@@ -613,6 +628,8 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     a.partials = pl->dPartials.p;
     a.jobOut = static_cast<double*>(pl->dJobOut.p);
     a.tileCounter = static_cast<unsigned long long*>(pl->dCounter.p);
+    a.chunkJob = static_cast<const int*>(pl->dChunkJob.p);
+    a.nChunks = pl->nChunks;
     a.recIds = rIds;
     a.recCount = rCnt;
     a.recL = rL;

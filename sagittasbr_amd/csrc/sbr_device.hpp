@@ -18,6 +18,7 @@ constexpr int kMaxStack4 = 96;
 constexpr int kMaxDepth4 = kMaxStack4 / (kWide - 1);
 constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a wide record
 constexpr int kCullBoxes = 64;          // boxes of the per-tile emptiness test (one per lane)
+constexpr int kChunkTiles = 64;         // tiles per entry of the tile -> job table
 
 // One sigma value (mesh, f, theta, phi) ready for the device. Frame and
 // physics constants are in the working precision R; kd / twoPid carry the
@@ -86,6 +87,10 @@ template <typename R> struct TraceArgs {
     void* partials;          // one accumulator per 16x16 ray pass
     double* jobOut;          // 2 doubles per job
     unsigned long long* tileCounter;  // persistent launch: next tile to take
+    // job of the first tile of each chunk of kChunkTiles tiles (the job of
+    // a tile lies between its chunk's entry and the next chunk's)
+    const int* chunkJob;
+    int64_t nChunks;
     // per-ray records (single-job debug launches)
     int32_t* recIds;
     int32_t* recCount;

@@ -636,19 +636,7 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
     unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<uint64_t>(v) >> 32));
     return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
 }
-__device__ __forceinline__ unsigned uniform32(unsigned v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__device__ __forceinline__ int find_job(const int64_t* base, int stride, int n, int64_t tile) {
-    // jobs are sorted by tileBase; binary search the last job with base <= tile
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        int64_t b = *reinterpret_cast<const int64_t*>(reinterpret_cast<const char*>(base) + size_t(mid) * stride);
-        if (b <= tile) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 // ------------------------------------------------------------ ray paths
 template <typename R> struct PathOut {
@@ -1020,7 +1008,19 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     if (PERSIST) tile = uniform64(sNext);
     while (tile < a.totalTiles) {
 
-        const int jid = find_job(&a.jobs[0].tileBase, sizeof(Job<R>), a.nJobs, tile);
+        // the last job whose first tile is <= tile, searched between the
+        // jobs of this tile's chunk and of the next chunk
+        int jid;
+        {
+            const int64_t c = tile / kChunkTiles;
+            int lo = a.chunkJob[c], hi = c + 1 < a.nChunks ? a.chunkJob[c + 1] : a.nJobs - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (a.jobs[mid].tileBase <= tile) lo = mid;
+                else hi = mid - 1;
+            }
+            jid = lo;
+        }
         const Job<R>& J = a.jobs[jid];
         const Mesh<R>& M = a.meshes[J.mesh];
         const int tl = static_cast<int>(tile - J.tileBase);
