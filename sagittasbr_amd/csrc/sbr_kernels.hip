@@ -445,8 +445,9 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     float hx = m.rootHi4[0] - cx, hy = m.rootHi4[1] - cy, hz = m.rootHi4[2] - cz;
     // (L1 norms bound the Euclidean ones)
     float tFar = (fabsf(o.x - cx) + fabsf(o.y - cy) + fabsf(o.z - cz) + hx + hy + hz) * (1.0f + 4.0f * kEps32);
-    // position and distance error bounds of the FP32 box tests over the query
-    float dx = ex + ed * tFar + K * kEps32 * (o1 + tFar);
+    // position and distance error bounds of the FP32 box tests over the
+    // query (the second tFar is the reciprocal's error beyond rounding)
+    float dx = ex + ed * tFar + K * kEps32 * (o1 + 2.0f * tFar);
     float tErr = dx + 4.0f * K * kEps32 * tFar;
     double closest = 1e20, second = 1e20;
     int hit = -1;
@@ -834,9 +835,13 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
     V3<float> of{float(o.x + shift * d.x), float(o.y + shift * d.y), float(o.z + shift * d.z)};
     V3<float> df{float(d.x), float(d.y), float(d.z)};
     // finite slab reciprocals: components below 1e-30 are taken as +-1e-30,
-    // a direction change far inside the direction error bound
+    // a direction change far inside the direction error bound. The hardware
+    // reciprocal (relative error up to 2^-23, one unit more than a rounded
+    // division) moves a slab distance t by up to 2^-23 t: the query's
+    // position bound carries the extra 2^-24 t of it.
     auto away = [](float c) { return copysignf(fmaxf(fabsf(c), 1e-30f), c); };
-    V3<float> inv{1.0f / away(df.x), 1.0f / away(df.y), 1.0f / away(df.z)};
+    V3<float> inv{__builtin_amdgcn_rcpf(away(df.x)), __builtin_amdgcn_rcpf(away(df.y)),
+                  __builtin_amdgcn_rcpf(away(df.z))};
     const float ex = K * kEps32 * (fabsf(of.x) + fabsf(of.y) + fabsf(of.z) + 1e-3f);
     const float ed = 2.0f * K * kEps32;
     double t;
