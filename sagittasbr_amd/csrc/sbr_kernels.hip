@@ -25,7 +25,8 @@
 // Arithmetic: FP32 code uses fmaf at fixed places (dot, cross, ray origin,
 // hit point, reflection, offset); FP64 code uses no fused operations. The
 // file is compiled with -ffp-contract=off so nothing else is fused. Division
-// and square root are IEEE correctly rounded.
+// and square root are IEEE correctly rounded (the mixed mode's FP32 slab
+// reciprocals excepted: their error is part of its box bound).
 //
 // Mixed mode (MODE 1): FP32 box traversal of a four-wide tree against an
 // FP32 copy of the FP64 reference ray, every box grown by that copy's
@@ -1514,7 +1515,7 @@ template <typename R> int trace_occupancy(int mode, int sd, bool pool) {
 template int trace_occupancy<float>(int, int, bool);
 template int trace_occupancy<double>(int, int, bool);
 
-// Size in bytes of one per-tile partial for (precision mode).
+// Size in bytes of one per-pass partial for (precision mode).
 size_t partial_bytes(bool fp64, int mode) {
     if (fp64) return sizeof(Acc<double, 0>);
     return mode == 1 ? sizeof(Acc<float, 1>) : sizeof(Acc<float, 0>);
