@@ -223,6 +223,30 @@ def test_mixed_undecided_queries(ctx, tri_duct, scene):
         ctx.destroy_mesh(mesh)
 
 
+@pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_FP64, S.SBR_MIXED])
+def test_job_sums_independent_of_neighbours(ctx, tri_duct, prec):
+    """Each job's sum comes from its own pass partials in its own grid order:
+    a job with no tiles (max_bounces 0) gives exactly zero, and the other
+    jobs' sums are the same to the bit with or without it and in any order."""
+    v, f, mesh, s32, s64 = tri_duct
+    p = S.SweepParams.make(**dict(C2S, theta_samples=3, phi_samples=3))
+    jobs = list(S.sweep_jobs(p, mesh, prec))
+    _step(f"precision {prec} plain")
+    z0, _, _ = ctx.sweep(mesh, p, prec, S.SBR_BATCHED)
+    empty = S.Job(jobs[0].mesh, 0, jobs[0].n_x, jobs[0].n_y, jobs[0].freq, jobs[0].theta_deg, jobs[0].phi_deg,
+                  jobs[0].grid_size, jobs[0].reflection_const)
+    mixed = [jobs[4], empty] + jobs[::-1]
+    _step(f"precision {prec} with an empty job, reversed")
+    pl = ctx.plan((S.Job * len(mixed))(*mixed), prec, S.SBR_BATCHED)
+    pl.run()
+    pl.wait()
+    z, _, _ = pl.results()
+    pl.close()
+    assert z[1] == 0
+    assert z[0] == z0[4]
+    assert np.array_equal(z[2:], z0[::-1])
+
+
 def test_tri_duct_sigma_fp64_and_methods(ctx, tri_duct):
     v, f, mesh, s32, s64 = tri_duct
     kw = dict(C2S, theta_samples=4, phi_samples=4, theta_end=90, phi_end=90)
