@@ -133,7 +133,7 @@ struct sbr_plan {
     DevBuf dChunkJob;  // job of the first tile of each kChunkTiles chunk
     int64_t nChunks = 0;
     int maxDepth = 0;
-    sbr_tuning tune{0, 0, 0, 0, 0};
+    sbr_tuning tune{0, 0, 0, 0, 0, 0};
     int tileSide = 32;
     DevBuf dJobs, dPartials, dJobOut, dCounter, dCounters;
     // mixed mode
