@@ -197,6 +197,8 @@ def adjoint_workload(args):
     jb = wl["job"]
     jobs = S.direction_jobs(mid, th, ph, jb["freq"], jb["grid_size"], jb["n_x"], jb["n_y"], jb["max_bounces"])
     plan = ctx.plan(jobs, S.SBR_MIXED, S.SBR_BATCHED)
+    tuning = plan.autotune(sample_jobs=0, repeats=1)
+    log(f"tuning {tuning}")
     ms, _ = timed(plan, max(args.steps, 1), args.warmup)
     z, sig, db = plan.results()
     plan.close()
@@ -213,7 +215,7 @@ def adjoint_workload(args):
         "value": round(n / (ms / 1000.0), 2), "unit": "angles/s", "n_gpus": 1, "steps": max(args.steps, 1),
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (procedural meshes)",
-        "config": {"workload": wl["desc"], "method": "batched", "precision": "mixed"},
+        "config": {"workload": wl["desc"], "method": "batched", "precision": "mixed", "tuning": tuning},
         "adjoint": {"seconds": round(adj_s, 4), "directions_per_s": round(n / adj_s, 2),
                     "gradient_values": int(dxyz.size), "nonzero_vertex_coordinates": int(np.count_nonzero(dxyz)),
                     "sigma_matches_forward": bool(np.allclose(sg, sig, rtol=1e-12)),
