@@ -651,6 +651,7 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
         if (record) {
             // every job's tiles in one grid (each block finds its job); job
             // i's records follow those of jobs 0..i-1
+            if (pl->totalTiles > INT32_MAX) return fail(&ctx->err, "record run too large for one launch");
             a.tileOffset = 0;
             HIP_TRY(&ctx->err,
                     launch_trace<R>(a, mode, sd, true, count, false, pool, ww, int(pl->totalTiles), s));
