@@ -1,6 +1,6 @@
-"""CPU tests of the drop-in surface: OBJ / glTF / GLB loading and the
-RT-RCS command line up to the device step (config keys, CLI rules, error
-exits)."""
+"""Tests of the drop-in surface: OBJ / glTF / GLB loading and the RT-RCS
+command line up to the device step (config keys, CLI rules, error exits) on
+the CPU, and one RT-RCS sweep end to end on the GPU."""
 import base64
 import json
 import os
@@ -164,3 +164,63 @@ def test_cli_error_exits(tmp_path):
     assert r.returncode == 1
     r = _run(["--model"], tmp_path)
     assert r.returncode == 1 and "--model requires a file path" in r.stderr
+
+
+def _read_csv(path):
+    lines = open(path).read().splitlines()
+    head = [x for x in lines if x.startswith("#")]
+    assert lines[len(head)] == "theta,phi,rcs_m2,rcs_dbsm"
+    rows = np.array([[float(t) for t in x.split(",")] for x in lines[len(head) + 1:]])
+    return head, rows
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(RTRCS), reason="RT-RCS not built")
+def test_cli_sweep_csv_matches_oracle(tmp_path):
+    """RT-RCS end to end on the GPU: an OBJ model and a config.txt in, the
+    rcs_results.csv rows out (rtRcsEntry.cu:298-302, sweep.cu:540-544),
+    checked against the FP64 oracle sweep of the same loaded vertices. The
+    CSV holds 6 significant digits, so sigma is compared to 6e-6 relative."""
+    import oracle_lib as O
+
+    v, f = M.trihedral_duct_scene()
+    M.write_obj(str(tmp_path / "scene.obj"), v, f)
+    lv, lf = S.load_model(str(tmp_path / "scene.obj"))
+    kw = dict(theta_start=0, theta_end=90, theta_samples=4, phi_start=0, phi_end=90, phi_samples=5, grid_size=1.6,
+              n_x=160, n_y=160, max_bounces=10, freq=10e9)
+    (tmp_path / "config.txt").write_text("".join(f"{k} {kw[k]}\n" for k in kw if k != "freq") + "freq 1e10\n")
+    n = kw["theta_samples"] * kw["phi_samples"]
+    th = np.repeat(np.linspace(0, 90, 4), 5)
+    ph = np.tile(np.linspace(0, 90, 5), 4)
+
+    def sigma_db(z):
+        s = 4 * np.pi * np.abs(z) ** 2
+        return s, 10 * np.log10(np.maximum(s, 1e-10))
+
+    for prec, method, scale in (("fp64", "batched", 1.0), ("fp64", "per_angle", 2.0), ("mixed", "batched", 1.0),
+                                ("fp32", "batched", 1.0)):
+        print(f"[cli] precision {prec} method {method} scale {scale}", flush=True)
+        if prec == "fp32":
+            # the oracle's FP32 twin traces every ray as the FP32 kernel does;
+            # only the order of the FP32 sums differs
+            z, _ = O.Scene(lv, lf, 0, scale=scale).sweep(O.Params(**kw), fused=True)
+        else:
+            z, _ = O.Scene(lv, lf, 1, scale=scale).sweep(O.Params(**kw))
+        s_ref, db_ref = sigma_db(z)
+        r = _run(["--model", "scene.obj", "--model-scale", str(scale), "--precision", prec, "--method", method],
+                 tmp_path)
+        assert r.returncode == 0, r.stderr[-2000:]
+        head, rows = _read_csv(tmp_path / "rcs_results.csv")
+        assert head == ["# Frequency: 1e+10", "# Grid: 160x160", "# GridSize: 1.6", "# ThetaSamples: 4",
+                        "# PhiSamples: 5"]
+        assert rows.shape == (n, 4)
+        np.testing.assert_allclose(rows[:, 0], th, atol=1e-4)
+        np.testing.assert_allclose(rows[:, 1], ph, atol=1e-4)
+        if prec == "fp32":
+            # |sqrt(sigma) - sqrt(sigma_ref)| <= sqrt(4 pi) |z - z_ref|: the
+            # amplitudes agree to the rounding of two FP32 sums
+            amp, amp_ref = np.sqrt(rows[:, 2]), np.sqrt(s_ref)
+            assert np.max(np.abs(amp - amp_ref)) <= 1e-4 * amp_ref.max()
+            continue
+        np.testing.assert_allclose(rows[:, 2], s_ref, rtol=6e-6, atol=1e-12 * s_ref.max())
+        np.testing.assert_allclose(rows[:, 3], db_ref, atol=2e-4)
