@@ -1,6 +1,7 @@
 """GPU parity at the benchmark sizes (sphere: 360 angles of 250 x 250 rays;
 trihedral+duct: 8,281 angles of 535 x 535 rays, 10 bounces; airframe:
-3.2M triangles at 1, 3 and 10 GHz, every 31st of its 33,480 angles).
+3.2M triangles at 1, 3 and 10 GHz, every 31st of its 33,480 angles; 64
+uneven bodies: all 23,040 jobs in one plan, eager and hipGraph replays).
 
 At these sizes the oracle checks a seeded sample of angles, and the rest is
 held by size-independent properties:
@@ -108,3 +109,44 @@ def test_full_airframe_sample(ctx):
     _step(f"mixed per-angle, {len(sub)} of the jobs reversed")
     zsp = _run(ctx, sub, S.SBR_MIXED, S.SBR_PER_ANGLE)
     assert np.array_equal(zsp, zs)
+
+
+def test_full_uneven_batch(ctx):
+    """64 bodies of 5k-400k triangles, each with its own frequency and grid,
+    360 angles each: all 23,040 jobs in one plan, replayed from a hipGraph."""
+    wl = W.c4()
+    print(f"[full] c4: {wl['desc']}", flush=True)
+    mids = [ctx.mesh(v, f) for v, f in wl["meshes"]]
+
+    def jobs(prec):
+        out = []
+        for mi, kw in wl["sweeps"]:
+            js = S.sweep_jobs(S.SweepParams.make(**kw), mids[mi], prec)
+            out.extend(js[k] for k in range(len(js)))
+        return out
+
+    jm, j64 = jobs(S.SBR_MIXED), jobs(S.SBR_FP64)
+    _step(f"fp64 batched, {len(j64)} jobs")
+    z64 = _run(ctx, j64, S.SBR_FP64, S.SBR_BATCHED)
+    # three seeded jobs against the FP64 oracle, each on its own body's scene
+    for k in np.sort(np.random.default_rng(7).choice(len(j64), 3, replace=False)):
+        mi, kw = wl["sweeps"][k // 360]
+        v, f = wl["meshes"][mi]
+        _step(f"fp64 oracle, job {k} (body {mi}, {len(f)} triangles, angle {k % 360})")
+        ref, _ = O.Scene(v, f, 1).sweep(O.Params(**kw), np.array([k % 360], np.int32))
+        body = z64[360 * (k // 360):360 * (k // 360 + 1)]
+        np.testing.assert_allclose(z64[k], ref[0], rtol=1e-9, atol=1e-9 * np.abs(body).max())
+    _step(f"mixed batched, {len(jm)} jobs, eager")
+    zm = _run(ctx, jm, S.SBR_MIXED, S.SBR_BATCHED)
+    _mixed_close(zm, z64)
+    _step("mixed batched, hipGraph replays")
+    pl = ctx.plan((S.Job * len(jm))(*jm), S.SBR_MIXED, S.SBR_BATCHED)
+    try:
+        pl.tune(0, 8, 32, 1, 2, 0)
+        for _ in range(3):
+            pl.run()
+            pl.wait()
+            zg, _, _ = pl.results()
+            assert np.array_equal(zg, zm)
+    finally:
+        pl.close()
