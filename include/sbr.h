@@ -189,8 +189,8 @@ int sbr_plan_set_tuning(sbr_plan* plan, const sbr_tuning* t);
 int sbr_plan_run(sbr_plan* plan);  /* asynchronous on the context stream */
 int sbr_plan_wait(sbr_plan* plan);
 /* Device time of the last run's ray-path kernel launch(es) in ms (hipEvents
- * on the plan's stream; under hipGraph replay the whole replay, output
- * clearing included). */
+ * on the plan's stream; under hipGraph replay the whole replay, the counter
+ * clearing and the job-sum kernel included). */
 int sbr_plan_last_ms(sbr_plan* plan, double* ms);
 /* Per-job complex sum (2 doubles per job), sigma = 4*pi*|sum|^2 and dBsm
  * (src/app/sweep.cu:228-231); any output may be NULL. */
