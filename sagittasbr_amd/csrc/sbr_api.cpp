@@ -117,6 +117,9 @@ struct sbr_ctx {
     std::vector<std::unique_ptr<MeshEntry>> meshes;
     DevBuf meshTable32, meshTable64;
     bool tableDirty32 = true, tableDirty64 = true;
+    // bumped whenever a mesh table moves to a new device address: a plan's
+    // captured hipGraph holds the address it was captured with
+    uint64_t tableGen = 0;
     std::set<sbr_plan*> plans;  // live plans, orphaned when the context goes
 };
 
@@ -142,6 +145,7 @@ struct sbr_plan {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
+    uint64_t graphTableGen = 0;  // ctx->tableGen when the graph was captured
     bool ran = false;
 };
 
@@ -178,8 +182,12 @@ void widen_boxes4(GpuBvh4& g) {
     }
 }
 
+template <typename R> int upload_table(sbr_ctx* ctx);
+
 // Triangle soup in R (src/app/rtRcsEntry.cu:118-148): float position
-// converted to R, times the model scale in R.
+// converted to R, times the model scale in R. The context's mesh table of
+// that precision is uploaded again at once, so it is current after every
+// build.
 template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
     PrecMesh<R>& pm = prec_of<R>(m);
     if (pm.built) return 0;
@@ -254,7 +262,7 @@ template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
     pm.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (sizeof(R) == 4) ctx->tableDirty32 = true;
     else ctx->tableDirty64 = true;
-    return 0;
+    return upload_table<R>(ctx);
 }
 
 template <typename R> int upload_table(sbr_ctx* ctx) {
@@ -296,7 +304,12 @@ template <typename R> int upload_table(sbr_ctx* ctx) {
         }
         tab[i] = mm;
     }
-    HIP_TRY(&ctx->err, buf.ensure(sizeof(Mesh<R>) * std::max<size_t>(tab.size(), 1)));
+    const void* before = buf.p;
+    const size_t need = sizeof(Mesh<R>) * std::max<size_t>(tab.size(), 1);
+    // a larger table replaces the old one: let runs still reading it finish
+    if (buf.p && need > buf.bytes) HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(&ctx->err, buf.ensure(need));
+    if (buf.p != before) ++ctx->tableGen;
     HIP_TRY(&ctx->err, hipMemcpy(buf.p, tab.data(), sizeof(Mesh<R>) * tab.size(), hipMemcpyHostToDevice));
     dirty = false;
     return 0;
@@ -437,11 +450,36 @@ template <typename R> void launch_coords(Job<R>& J) {
 
 constexpr float kErrSafety = 2.0f;  // safety factor on the mixed mode's forward error bounds
 
-template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
+// Host side of a plan: device jobs (frames, constants, culled tile
+// rectangles, pass grids), the tile -> job table and the partial count.
+template <typename R> struct HostPlan {
+    std::vector<Job<R>> jobs;
+    std::vector<Job<double>> jobs64;  // mixed mode
+    std::vector<int> chunkJob;
+    int64_t passes = 0;
+};
+
+// Device buffers of a plan run (the plan's own, or caller-provided storage).
+struct DevPlan {
+    void* jobs = nullptr;
+    void* partials = nullptr;
+    double* jobOut = nullptr;
+    unsigned long long* tileCounter = nullptr;
+    unsigned long long* counters = nullptr;  // 16 words
+    const int* chunkJob = nullptr;
+    void* jobs64 = nullptr;
+    int8_t* recFlag = nullptr;
+};
+
+// Fills hp and the plan's tile bookkeeping. noAlloc: fail instead of
+// building a tree that is not built yet (building allocates device memory).
+template <typename R> int plan_host(sbr_plan* pl, bool record, bool noAlloc, HostPlan<R>& hp) {
     sbr_ctx* ctx = pl->ctx;
     const bool mixed = pl->precision == SBR_MIXED;
-    std::vector<Job<R>> jobs(pl->nJobs);
-    std::vector<Job<double>> jobs64(mixed ? pl->nJobs : 0);
+    std::vector<Job<R>>& jobs = hp.jobs;
+    std::vector<Job<double>>& jobs64 = hp.jobs64;
+    jobs.assign(pl->nJobs, Job<R>{});
+    jobs64.assign(mixed ? pl->nJobs : 0, Job<double>{});
     pl->tileBase.assign(pl->nJobs, 0);
     pl->nTiles.assign(pl->nJobs, 0);
     int64_t base = 0;
@@ -458,6 +496,9 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         if (int64_t(j.n_x) * j.n_y > 0x7fffffffll)
             return fail(&ctx->err, "job " + std::to_string(i) + ": n_x*n_y exceeds the int ray index");
         MeshEntry& m = *ctx->meshes[j.mesh];
+        if (noAlloc && !prec_of<R>(m).built)
+            return fail(&ctx->err, std::string("mesh ") + std::to_string(j.mesh) + " has no " +
+                                       (sizeof(R) == 8 ? "FP64" : "FP32") + " tree yet (building it allocates)");
         if (ensure_built<R>(ctx, m)) return 1;
         PrecMesh<R>& pm = prec_of<R>(m);
         maxDepth = std::max(maxDepth, pm.bvh.depth);
@@ -472,6 +513,9 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
                     return fail(&ctx->err, "mixed mode: four-wide tree depth " + std::to_string(pm.gpu4.depth) +
                                                " exceeds " + std::to_string(kMaxDepth4));
             }
+            if (noAlloc && !m.f64.built)
+                return fail(&ctx->err, "mesh " + std::to_string(j.mesh) +
+                                           " has no FP64 tree yet (the mixed mode needs it; building it allocates)");
             if (ensure_built<double>(ctx, m)) return 1;
             Job<double> D{};
             make_frame<double>(j.theta_deg, j.phi_deg, j.grid_size, D.llc, D.uv, D.vv, D.dir);
@@ -510,6 +554,22 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
         J.maxB = j.max_bounces;
         J.mesh = j.mesh;
         launch_coords<R>(J);
+        {
+            // FP32 rounding of the tile test's a = X . cullA - cullA[3] over
+            // the mesh's boxes: three products and three sums (4 eps of the
+            // terms' magnitudes) plus the rounding of cullA itself (1 eps),
+            // doubled; 64 eps cover the ray-centre coordinates and the FP32
+            // frame of the mixed mode's FP64 rays
+            double sa = std::fabs(double(J.cullA[3])), sb = std::fabs(double(J.cullB[3]));
+            for (int k = 0; k < 3; ++k) {
+                const double mk = std::max(std::fabs(double(pm.gpu.rootLo[k])), std::fabs(double(pm.gpu.rootHi[k])));
+                sa += mk * std::fabs(double(J.cullA[k]));
+                sb += mk * std::fabs(double(J.cullB[k]));
+            }
+            constexpr double e32 = 5.9604644775390625e-8;
+            J.cullErrA = float(10.0 * e32 * sa + 64.0 * e32);
+            J.cullErrB = float(10.0 * e32 * sb + 64.0 * e32);
+        }
         int i0 = 0, i1 = j.n_x - 1, j0 = 0, j1 = j.n_y - 1;
         bool any = true;
         if (!record) any = cull_rect<R>(J, pm.gpu.rootLo, pm.gpu.rootHi, i0, i1, j0, j1);
@@ -547,32 +607,57 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
     {
         // tile -> job table: the last job whose first tile is <= chunk * kChunkTiles
         pl->nChunks = (base + kChunkTiles - 1) / kChunkTiles;
-        std::vector<int> cj(std::max<int64_t>(pl->nChunks, 1), 0);
+        std::vector<int>& cj = hp.chunkJob;
+        cj.assign(std::max<int64_t>(pl->nChunks, 1), 0);
         int jj = 0;
         for (int64_t c = 0; c < pl->nChunks; ++c) {
             const int64_t t = c * kChunkTiles;
             while (jj + 1 < pl->nJobs && pl->tileBase[jj + 1] <= t) ++jj;
             cj[size_t(c)] = jj;
         }
-        HIP_TRY(&ctx->err, pl->dChunkJob.ensure(cj.size() * sizeof(int)));
-        HIP_TRY(&ctx->err, hipMemcpy(pl->dChunkJob.p, cj.data(), cj.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     pl->maxDepth = maxDepth;
+    hp.passes = passes;
+    return 0;
+}
+
+// Host side, then the plan's own device buffers.
+template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
+    sbr_ctx* ctx = pl->ctx;
+    const bool mixed = pl->precision == SBR_MIXED;
+    HostPlan<R> hp;
+    if (plan_host<R>(pl, record, false, hp)) return 1;
     if (upload_table<R>(ctx)) return 1;
     if (mixed && upload_table<double>(ctx)) return 1;
     const int mode = mixed ? 1 : 0;
+    const std::vector<int>& cj = hp.chunkJob;
+    HIP_TRY(&ctx->err, pl->dChunkJob.ensure(cj.size() * sizeof(int)));
+    HIP_TRY(&ctx->err, hipMemcpy(pl->dChunkJob.p, cj.data(), cj.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(&ctx->err, pl->dJobs.ensure(sizeof(Job<R>) * std::max(pl->nJobs, 1)));
-    HIP_TRY(&ctx->err, hipMemcpy(pl->dJobs.p, jobs.data(), sizeof(Job<R>) * pl->nJobs, hipMemcpyHostToDevice));
-    HIP_TRY(&ctx->err, pl->dPartials.ensure(partial_bytes(sizeof(R) == 8, mode) * std::max<int64_t>(passes, 1)));
+    HIP_TRY(&ctx->err, hipMemcpy(pl->dJobs.p, hp.jobs.data(), sizeof(Job<R>) * pl->nJobs, hipMemcpyHostToDevice));
+    HIP_TRY(&ctx->err, pl->dPartials.ensure(partial_bytes(sizeof(R) == 8, mode) * std::max<int64_t>(hp.passes, 1)));
     HIP_TRY(&ctx->err, pl->dJobOut.ensure(sizeof(double) * 2 * std::max(pl->nJobs, 1)));
     HIP_TRY(&ctx->err, pl->dCounter.ensure(sizeof(unsigned long long)));
     HIP_TRY(&ctx->err, pl->dCounters.ensure(sizeof(unsigned long long) * 16));
     if (mixed) {
         HIP_TRY(&ctx->err, pl->dJobs64.ensure(sizeof(Job<double>) * std::max(pl->nJobs, 1)));
         HIP_TRY(&ctx->err,
-                hipMemcpy(pl->dJobs64.p, jobs64.data(), sizeof(Job<double>) * pl->nJobs, hipMemcpyHostToDevice));
+                hipMemcpy(pl->dJobs64.p, hp.jobs64.data(), sizeof(Job<double>) * pl->nJobs, hipMemcpyHostToDevice));
     }
     return 0;
+}
+
+DevPlan dev_of(sbr_plan* pl) {
+    DevPlan d;
+    d.jobs = pl->dJobs.p;
+    d.partials = pl->dPartials.p;
+    d.jobOut = static_cast<double*>(pl->dJobOut.p);
+    d.tileCounter = static_cast<unsigned long long*>(pl->dCounter.p);
+    d.counters = static_cast<unsigned long long*>(pl->dCounters.p);
+    d.chunkJob = static_cast<const int*>(pl->dChunkJob.p);
+    d.jobs64 = pl->dJobs64.p;
+    d.recFlag = static_cast<int8_t*>(pl->dRecFlag.p);
+    return d;
 }
 
 int default_sd(int /*depth*/) { return 8; }
@@ -609,8 +694,8 @@ void release_plan_device(sbr_plan* pl) {
 
 // Enqueue one run: clear outputs/counters with fill kernels (no memset
 // nodes, so the sequence is safe to capture), then the trace launch(es).
-template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, int32_t* rIds, int32_t* rCnt, double* rL,
-                                       double* rReIm, bool timed) {
+template <typename R> int plan_enqueue(sbr_plan* pl, const DevPlan& d, bool record, bool count, int32_t* rIds,
+                                       int32_t* rCnt, double* rL, double* rReIm, bool timed) {
     sbr_ctx* ctx = pl->ctx;
     const int mode = pl->precision == SBR_MIXED ? 1 : 0;
     const int sd = pl->tune.lds_stack > 0 ? pl->tune.lds_stack : default_sd(pl->maxDepth);
@@ -619,32 +704,32 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     // gains in lane use, so the pool is opt-in (the autotuner tries both)
     const bool pool = pl->tune.ray_pool == 1 && !record;
     TraceArgs<R> a{};
-    a.jobs = static_cast<const Job<R>*>(pl->dJobs.p);
+    a.jobs = static_cast<const Job<R>*>(d.jobs);
     a.meshes = static_cast<const Mesh<R>*>(sizeof(R) == 4 ? ctx->meshTable32.p : ctx->meshTable64.p);
     a.nJobs = pl->nJobs;
     a.tileSide = pl->tileSide;
     const bool ww = (pl->tune.traversal ? pl->tune.traversal : default_traversal()) == 2;
     a.totalTiles = pl->totalTiles;
-    a.partials = pl->dPartials.p;
-    a.jobOut = static_cast<double*>(pl->dJobOut.p);
-    a.tileCounter = static_cast<unsigned long long*>(pl->dCounter.p);
-    a.chunkJob = static_cast<const int*>(pl->dChunkJob.p);
+    a.partials = d.partials;
+    a.jobOut = d.jobOut;
+    a.tileCounter = d.tileCounter;
+    a.chunkJob = d.chunkJob;
     a.nChunks = pl->nChunks;
     a.recIds = rIds;
     a.recCount = rCnt;
     a.recL = rL;
     a.recReIm = rReIm;
-    a.counters = static_cast<unsigned long long*>(pl->dCounters.p);
+    a.counters = d.counters;
     const bool mixed = pl->precision == SBR_MIXED;
     if (mixed) {
-        a.jobs64 = static_cast<const Job<double>*>(pl->dJobs64.p);
+        a.jobs64 = static_cast<const Job<double>*>(d.jobs64);
         a.meshes64 = static_cast<const Mesh<double>*>(ctx->meshTable64.p);
         a.errK = kErrSafety;
-        a.recFlag = static_cast<int8_t*>(pl->dRecFlag.p);
+        a.recFlag = d.recFlag;
     }
     hipStream_t s = ctx->stream;
-    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounter.p), 2, s));
-    HIP_TRY(&ctx->err, launch_zero(static_cast<unsigned*>(pl->dCounters.p), 32, s));
+    HIP_TRY(&ctx->err, launch_zero(reinterpret_cast<unsigned*>(d.tileCounter), 2, s));
+    HIP_TRY(&ctx->err, launch_zero(reinterpret_cast<unsigned*>(d.counters), 32, s));
     // the timed region covers the trace launches only
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
     if (pl->totalTiles > 0) {
@@ -673,7 +758,7 @@ template <typename R> int plan_enqueue(sbr_plan* pl, bool record, bool count, in
     }
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
     // job sums from the pass partials, in each job's pass-grid order
-    HIP_TRY(&ctx->err, launch_finalize<R>(a.jobs, pl->nJobs, mode, pl->dPartials.p, a.jobOut, s));
+    HIP_TRY(&ctx->err, launch_finalize<R>(a.jobs, pl->nJobs, mode, d.partials, a.jobOut, s));
     return 0;
 }
 
@@ -684,9 +769,12 @@ template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int
     for (int id : pl->meshIds)
         if (!ctx->meshes[id]) return fail(&ctx->err, "a mesh of this plan was destroyed");
     if (pl->tune.use_graph && !record && !count) {
+        // the graph holds the mesh table addresses of its capture: capture
+        // again after a table moved (a later mesh grew it)
+        if (pl->graphExec && pl->graphTableGen != ctx->tableGen) drop_graph(pl);
         if (!pl->graphExec) {
             HIP_TRY(&ctx->err, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            int rc = plan_enqueue<R>(pl, false, false, nullptr, nullptr, nullptr, nullptr, false);
+            int rc = plan_enqueue<R>(pl, dev_of(pl), false, false, nullptr, nullptr, nullptr, nullptr, false);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(s, &g);
             if (rc) {
@@ -696,13 +784,14 @@ template <typename R> int plan_launch(sbr_plan* pl, bool record, bool count, int
             if (e != hipSuccess) return fail(&ctx->err, std::string("graph capture failed: ") + hipGetErrorString(e));
             pl->graph = g;
             HIP_TRY(&ctx->err, hipGraphInstantiate(&pl->graphExec, g, nullptr, nullptr, 0));
+            pl->graphTableGen = ctx->tableGen;
         }
         // events bracket the whole replay (fill kernels included)
         HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
         HIP_TRY(&ctx->err, hipGraphLaunch(pl->graphExec, s));
         HIP_TRY(&ctx->err, hipEventRecord(pl->ev1, s));
     } else {
-        if (plan_enqueue<R>(pl, record, count, rIds, rCnt, rL, rReIm, true)) return 1;
+        if (plan_enqueue<R>(pl, dev_of(pl), record, count, rIds, rCnt, rL, rReIm, true)) return 1;
     }
     pl->ran = true;
     return 0;
@@ -847,6 +936,9 @@ int sbr_mesh_destroy(sbr_ctx* ctx, int mesh_id) {
     HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
     ctx->meshes[mesh_id].reset();
     ctx->tableDirty32 = ctx->tableDirty64 = true;
+    // tables keep their size: the destroyed entry is cleared in place
+    if (ctx->meshTable32.p && upload_table<float>(ctx)) return 1;
+    if (ctx->meshTable64.p && upload_table<double>(ctx)) return 1;
     return 0;
 }
 

@@ -35,6 +35,9 @@ template <typename R> struct alignas(16) Job {
     // launch-plane coordinates of a point X: a = X . cullA[0..2] - cullA[3]
     // (ray (i, j) sits at a = (i + 0.5) / nx), b likewise with cullB
     float cullA[4], cullB[4];
+    // bounds on the FP32 rounding of a and b over the mesh's boxes (the
+    // products and sums grow with the coordinates' size against the grid)
+    float cullErrA, cullErrB;
     int64_t tileBase;                // first global tile of this job
     // record runs: this job's first ray in the per-ray arrays and first
     // entry of the hit-face array (jobs one after another)

@@ -909,8 +909,9 @@ __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J,
 // (lane q tests box q; the result is wave-uniform). A box is reached only by
 // rays whose launch-plane coordinates fall in its projection along the ray
 // direction; the test compares that projection's bounding rectangle, widened
-// by half a ray spacing and 1e-5 of the grid (far above its FP32 rounding),
-// with the rectangle of ray centres. A tile none of whose rays can reach a
+// by half a ray spacing, 1e-5 of the grid and the host's bound on its FP32
+// rounding (which grows with the mesh's distance from the origin in grid
+// sizes), with the rectangle of ray centres. A tile none of whose rays can reach a
 // box contributes exactly zero and is not traced.
 template <typename R>
 __device__ __forceinline__ bool tile_reaches_mesh(const Job<R>& J, const Mesh<R>& M, int ia, int ib, int ja, int jb,
@@ -929,7 +930,7 @@ __device__ __forceinline__ bool tile_reaches_mesh(const Job<R>& J, const Mesh<R>
             bmin += fminf(pb, qb);
             bmax += fmaxf(pb, qb);
         }
-        const float ma = 1e-5f + 0.5f / float(J.nx), mb = 1e-5f + 0.5f / float(J.ny);
+        const float ma = 1e-5f + 0.5f / float(J.nx) + J.cullErrA, mb = 1e-5f + 0.5f / float(J.ny) + J.cullErrB;
         const float aLo = (float(ia) + 0.5f) / float(J.nx), aHi = (float(ib) + 0.5f) / float(J.nx);
         const float bLo = (float(ja) + 0.5f) / float(J.ny), bHi = (float(jb) + 0.5f) / float(J.ny);
         reach = amax >= aLo - ma && amin <= aHi + ma && bmax >= bLo - mb && bmin <= bHi + mb;

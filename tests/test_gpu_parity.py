@@ -311,6 +311,41 @@ def test_graph_replay_matches_eager(ctx, sphere, tri_duct):
         pl.close()
 
 
+def test_graph_replay_after_mesh_table_grows():
+    """A plan captured in a hipGraph holds the device address of the mesh
+    table; meshes created later grow the table to a new address. The next
+    replay must use the live table (captured again), not the freed one."""
+    c = S.Context(0)
+    try:
+        v, f = M.icosphere(3)
+        m0 = c.mesh(v, f)
+        kw = dict(C1, phi_samples=8, phi_end=70, n_x=96, n_y=96)
+        p = S.SweepParams.make(**kw)
+        for prec in (S.SBR_FP32, S.SBR_MIXED):
+            _step(f"graph capture, precision {prec}")
+            pl = c.plan(S.sweep_jobs(p, m0, prec), prec, S.SBR_BATCHED)
+            pl.tune(use_graph=1)
+            pl.run()
+            pl.wait()
+            z0, _, _ = pl.results()
+            for k in range(3):
+                # new meshes and a plan on each grow both mesh tables
+                vk, fk = M.icosphere(2 + k % 2)
+                mk = c.mesh(vk * (1.0 + 0.1 * k), fk)
+                other = c.plan(S.sweep_jobs(p, mk, S.SBR_MIXED)[:2], S.SBR_MIXED, S.SBR_BATCHED)
+                other.run()
+                other.wait()
+                other.close()
+                _step(f"replay after table growth {k}")
+                pl.run()
+                pl.wait()
+                z, _, _ = pl.results()
+                assert np.array_equal(z, z0)
+            pl.close()
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_MIXED, S.SBR_FP64])
 def test_tuning_variants(ctx, tri_duct, prec):
     """Partial sums are kept per 16x16 ray pass and reduced in a fixed order,
