@@ -346,6 +346,39 @@ def test_graph_replay_after_mesh_table_grows():
         c.close()
 
 
+@pytest.mark.parametrize("offset", [0.0, 4e3, 4e4])
+def test_tile_cull_far_offset_mesh(ctx, offset):
+    """The per-tile emptiness test skips tiles whose rays cannot reach the
+    mesh. Its FP32 rounding grows with the mesh's distance from the origin
+    in grid sizes; for a body far down the ray path the sums with the test
+    (a timed run) must equal those of a counting run, which traces every
+    ray."""
+    v, f = M.icosphere(3)
+    grid = 2.5
+    th, ph = np.radians(90.0), np.radians(20.0)
+    d = np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    vo = (v.astype(np.float64) - offset * grid * d).astype(np.float32)
+    mesh = ctx.mesh(vo, f)
+    # directions close enough that the far body stays inside the launch grid
+    # (its projection moves by offset * grid * angle)
+    dd = 2.0 if offset == 0 else float(np.degrees(0.3 / (offset * grid)))
+    p = S.SweepParams.make(theta_start=90 - dd, theta_end=90 + dd, theta_samples=3, phi_start=20 - dd,
+                           phi_end=20 + dd, phi_samples=3, grid_size=grid, n_x=256, n_y=256, max_bounces=3, freq=3e9)
+    for prec in (S.SBR_FP32, S.SBR_MIXED):
+        _step(f"offset {offset} grid sizes, precision {prec}")
+        pl = ctx.plan(S.sweep_jobs(p, mesh, prec), prec, S.SBR_BATCHED)
+        pl.tune(tile_rays=16)
+        pl.run()
+        pl.wait()
+        z_cull, _, _ = pl.results()
+        pl.count()
+        z_all, _, _ = pl.results()
+        pl.close()
+        assert np.all(np.abs(z_all) > 0)
+        assert np.array_equal(z_cull, z_all)
+    ctx.destroy_mesh(mesh)
+
+
 @pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_MIXED, S.SBR_FP64])
 def test_tuning_variants(ctx, tri_duct, prec):
     """Partial sums are kept per 16x16 ray pass and reduced in a fixed order,
