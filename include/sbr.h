@@ -215,6 +215,24 @@ int sbr_plan_autotune(sbr_plan* plan, int sample_jobs, int repeats, sbr_tuning* 
 int sbr_sweep(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precision, int method, double* reim,
               double* sigma, double* dbsm);
 
+/* Allocation-free form of sbr_sweep (batched method). No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It calls no hipMalloc, hipFree or other allocating routine: the
+ * job records, tile table, per-pass partials, job sums and counters live in
+ * one fixed device array of SBR_NOALLOC_SCRATCH_BYTES per device, held until
+ * the process ends. A sweep whose storage exceeds it fails with an error
+ * (split it, or use sbr_sweep). The mesh's trees must exist: the FP32 tree
+ * is built by sbr_mesh_create; SBR_FP64 and SBR_MIXED also need the FP64
+ * tree, built (allocating) by the first plan or sbr_mesh_info_get with
+ * SBR_FP64 on that mesh. Sums are bit-identical to sbr_sweep's. Every call on
+ * one device shares that device's array: calls that overlap on one device
+ * (other threads, other contexts on the device) race and must not be made.
+ * Blocks until the results are on the host. */
+#define SBR_NOALLOC_SCRATCH_BYTES (64u << 20)
+int sbr_sweep_noalloc(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precision, double* reim,
+                      double* sigma, double* dbsm);
+
 /* Per-ray debug trace of one angle of a sweep (the reference's
  * launchRaysMultiBounce outputs, src/cuda/rtKernels.cu:105-158, plus hit
  * triangle ids which the reference does not record). Rays are row-major

@@ -177,6 +177,7 @@ def lib():
         "sbr_plan_count": (C.c_int, [_P, C.POINTER(Counters)]),
         "sbr_plan_destroy": (C.c_int, [_P]),
         "sbr_sweep": (C.c_int, [_P, C.c_int, C.POINTER(SweepParams), C.c_int, C.c_int, _P, _P, _P]),
+        "sbr_sweep_noalloc": (C.c_int, [_P, C.c_int, C.POINTER(SweepParams), C.c_int, _P, _P, _P]),
         "sbr_trace_rays": (C.c_int, [_P, C.c_int, C.POINTER(SweepParams), C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     }
     for name, (res, args) in sig.items():
@@ -393,6 +394,17 @@ class Context:
         sig = np.zeros(n, np.float64)
         db = np.zeros(n, np.float64)
         _check(lib().sbr_sweep(self.h, mesh, C.byref(params), precision, method, _ptr(reim), _ptr(sig), _ptr(db)),
+               self.h)
+        return reim[0::2] + 1j * reim[1::2], sig, db
+
+    def sweep_noalloc(self, mesh, params: SweepParams, precision=SBR_FP32):
+        """Batched sweep through the allocation-free entry point (its device
+        storage is a fixed array; see sbr_sweep_noalloc in include/sbr.h)."""
+        n = params.n_angles
+        reim = np.zeros(2 * n, np.float64)
+        sig = np.zeros(n, np.float64)
+        db = np.zeros(n, np.float64)
+        _check(lib().sbr_sweep_noalloc(self.h, mesh, C.byref(params), precision, _ptr(reim), _ptr(sig), _ptr(db)),
                self.h)
         return reim[0::2] + 1j * reim[1::2], sig, db
 

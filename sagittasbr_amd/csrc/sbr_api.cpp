@@ -34,9 +34,13 @@ hipError_t launch_finalize(const Job<R>* jobs, int nJobs, int mode, const void* 
 size_t partial_bytes(bool fp64, int mode);
 hipError_t launch_adjoint(const AdjArgs& a, int blocks, hipStream_t s);
 template <typename R> int trace_occupancy(int mode, int sd, bool pool);
+hipError_t scratch_init(int device);
+void* scratch_ptr(int device);
 }  // namespace sbr
 
 using namespace sbr;
+
+static_assert(kScratchBytes == SBR_NOALLOC_SCRATCH_BYTES, "scratch size of the header and the kernels differ");
 
 namespace {
 
@@ -121,6 +125,7 @@ struct sbr_ctx {
     // captured hipGraph holds the address it was captured with
     uint64_t tableGen = 0;
     std::set<sbr_plan*> plans;  // live plans, orphaned when the context goes
+    std::string scratchErr;     // why the fixed scratch array has no address
 };
 
 struct sbr_plan {
@@ -857,6 +862,77 @@ template <typename R> void sigma_of(double re, double im, double* s, double* db)
     *db = double(R(10.0) * std::log10(std::fmax(sg, R(1e-10))));
 }
 
+// Job sums (2 doubles per job) -> caller outputs; any may be NULL.
+void write_results(int precision, const double* sums, int n, double* reim, double* sigma, double* dbsm) {
+    for (int i = 0; i < n; ++i) {
+        const double re = sums[2 * i], im = sums[2 * i + 1];
+        if (reim) {
+            reim[2 * i] = re;
+            reim[2 * i + 1] = im;
+        }
+        double s = 0, d = 0;
+        if (precision == SBR_FP32) sigma_of<float>(re, im, &s, &d);
+        else sigma_of<double>(re, im, &s, &d);
+        if (sigma) sigma[i] = s;
+        if (dbsm) dbsm[i] = d;
+    }
+}
+
+// Batched sweep with every device buffer carved out of the fixed scratch
+// array at base (no device allocation). The host plan is built first, so a
+// sweep too large for the array is refused before anything is launched.
+template <typename R> int sweep_in_scratch(sbr_plan* pl, void* base, std::vector<double>& sums) {
+    sbr_ctx* ctx = pl->ctx;
+    const bool mixed = pl->precision == SBR_MIXED;
+    HostPlan<R> hp;
+    if (plan_host<R>(pl, false, true, hp)) return 1;
+    if ((sizeof(R) == 4 ? ctx->tableDirty32 : ctx->tableDirty64) ||
+        !(sizeof(R) == 4 ? ctx->meshTable32.p : ctx->meshTable64.p) ||
+        (mixed && (ctx->tableDirty64 || !ctx->meshTable64.p)))
+        return fail(&ctx->err, "mesh tables are not on the device");
+    const int n = pl->nJobs;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+        return o;
+    };
+    const size_t oJobs = take(sizeof(Job<R>) * n);
+    const size_t oJobs64 = take(mixed ? sizeof(Job<double>) * n : 0);
+    const size_t oChunk = take(sizeof(int) * hp.chunkJob.size());
+    const size_t oPart = take(partial_bytes(sizeof(R) == 8, mixed ? 1 : 0) * size_t(hp.passes));
+    const size_t oOut = take(sizeof(double) * 2 * n);
+    const size_t oTile = take(sizeof(unsigned long long));
+    const size_t oCnt = take(sizeof(unsigned long long) * 16);
+    if (off > kScratchBytes)
+        return fail(&ctx->err, "sweep needs " + std::to_string(off) + " bytes of device scratch, more than the " +
+                                   std::to_string(kScratchBytes) +
+                                   "-byte array of the allocation-free sweep: split it, or use sbr_sweep");
+    unsigned char* b = static_cast<unsigned char*>(base);
+    DevPlan d;
+    d.jobs = b + oJobs;
+    d.jobs64 = mixed ? b + oJobs64 : nullptr;
+    d.chunkJob = reinterpret_cast<const int*>(b + oChunk);
+    d.partials = b + oPart;
+    d.jobOut = reinterpret_cast<double*>(b + oOut);
+    d.tileCounter = reinterpret_cast<unsigned long long*>(b + oTile);
+    d.counters = reinterpret_cast<unsigned long long*>(b + oCnt);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(&ctx->err, hipMemcpyAsync(d.jobs, hp.jobs.data(), sizeof(Job<R>) * n, hipMemcpyHostToDevice, s));
+    if (mixed)
+        HIP_TRY(&ctx->err,
+                hipMemcpyAsync(d.jobs64, hp.jobs64.data(), sizeof(Job<double>) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(&ctx->err, hipMemcpyAsync(b + oChunk, hp.chunkJob.data(), sizeof(int) * hp.chunkJob.size(),
+                                      hipMemcpyHostToDevice, s));
+    // the tile counter and the work counters are cleared by fill kernels in
+    // plan_enqueue; every pass partial is written by its tile
+    if (plan_enqueue<R>(pl, d, false, false, nullptr, nullptr, nullptr, nullptr, false)) return 1;
+    sums.assign(size_t(2) * n, 0.0);
+    HIP_TRY(&ctx->err, hipMemcpyAsync(sums.data(), d.jobOut, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(&ctx->err, hipStreamSynchronize(s));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -876,6 +952,10 @@ int sbr_ctx_create(int device, sbr_ctx** out) {
     HIP_TRY(nullptr, hipSetDevice(device));
     HIP_TRY(nullptr, hipDeviceGetAttribute(&ctx->cuCount, hipDeviceAttributeMultiprocessorCount, device));
     HIP_TRY(nullptr, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // device address of the allocation-free sweep's scratch array, looked up
+    // once per device with the device current
+    if (hipError_t e = scratch_init(device); e != hipSuccess)
+        ctx->scratchErr = std::string("scratch array address lookup failed: ") + hipGetErrorString(e);
     *out = ctx.release();
     return 0;
 }
@@ -1223,18 +1303,7 @@ int sbr_plan_results(sbr_plan* pl, double* reim, double* sigma, double* dbsm) {
     HIP_TRY(&ctx->err, hipMemcpyAsync(pl->out.data(), pl->dJobOut.p, sizeof(double) * 2 * pl->nJobs,
                                       hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
-    for (int i = 0; i < pl->nJobs; ++i) {
-        double re = pl->out[2 * i], im = pl->out[2 * i + 1];
-        if (reim) {
-            reim[2 * i] = re;
-            reim[2 * i + 1] = im;
-        }
-        double s = 0, d = 0;
-        if (pl->precision == SBR_FP32) sigma_of<float>(re, im, &s, &d);
-        else sigma_of<double>(re, im, &s, &d);
-        if (sigma) sigma[i] = s;
-        if (dbsm) dbsm[i] = d;
-    }
+    write_results(pl->precision, pl->out.data(), pl->nJobs, reim, sigma, dbsm);
     return 0;
 }
 
@@ -1353,6 +1422,36 @@ int sbr_sweep(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precisio
     if (!rc) rc = sbr_plan_results(pl, reim, sigma, dbsm);
     sbr_plan_destroy(pl);
     return rc;
+}
+
+int sbr_sweep_noalloc(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int precision, double* reim,
+                      double* sigma, double* dbsm) {
+    if (!ctx || !p) return fail(nullptr, "null argument");
+    if (!valid_prec(precision)) return fail(&ctx->err, "unknown precision");
+    if (p->theta_samples <= 0 || p->phi_samples <= 0) return fail(&ctx->err, "sample counts must be positive");
+    if (mesh_id < 0 || mesh_id >= int(ctx->meshes.size()) || !ctx->meshes[mesh_id])
+        return fail(&ctx->err, "bad or destroyed mesh id");
+    HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
+    void* base = scratch_ptr(ctx->device);
+    if (!base)
+        return fail(&ctx->err, ctx->scratchErr.empty() ? "no scratch array address for this device" : ctx->scratchErr);
+    const int n = p->theta_samples * p->phi_samples;
+    // a host-only plan: its device buffers stay empty, the run uses the array
+    sbr_plan pl;
+    pl.ctx = ctx;
+    pl.precision = precision;
+    pl.method = SBR_BATCHED;
+    pl.nJobs = n;
+    pl.tileSide = default_tile(SBR_BATCHED);
+    pl.jobs.resize(n);
+    if (sbr_sweep_jobs(p, mesh_id, precision, pl.jobs.data(), n)) return fail(&ctx->err, g_err);
+    pl.meshIds.push_back(mesh_id);
+    std::vector<double> sums;
+    const int rc = precision == SBR_FP64 ? sweep_in_scratch<double>(&pl, base, sums)
+                                         : sweep_in_scratch<float>(&pl, base, sums);
+    if (rc) return rc;
+    write_results(precision, sums.data(), n, reim, sigma, dbsm);
+    return 0;
 }
 
 int sbr_trace_rays(sbr_ctx* ctx, int mesh_id, const sbr_sweep_params* p, int angle, int precision, int32_t* hit_ids,

@@ -1,6 +1,7 @@
 // Plain data shared by the host driver and the HIP kernels.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 namespace sbr {
@@ -19,6 +20,8 @@ constexpr int kMaxDepth4 = kMaxStack4 / (kWide - 1);
 constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a wide record
 constexpr int kCullBoxes = 64;          // boxes of the per-tile emptiness test (one per lane)
 constexpr int kChunkTiles = 64;         // tiles per entry of the tile -> job table
+// Fixed device scratch of the allocation-free sweep (sbr_sweep_noalloc)
+constexpr size_t kScratchBytes = size_t(64) << 20;
 
 // One sigma value (mesh, f, theta, phi) ready for the device. Frame and
 // physics constants are in the working precision R; kd / twoPid carry the

@@ -40,6 +40,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <mutex>
 #include <type_traits>
 
 #include "sbr_device.hpp"
@@ -1515,6 +1516,40 @@ template <typename R> int trace_occupancy(int mode, int sd, bool pool) {
 }
 template int trace_occupancy<float>(int, int, bool);
 template int trace_occupancy<double>(int, int, bool);
+
+// ------------------------------------------------ fixed scratch memory
+// Device storage of the allocation-free sweep (sbr_sweep_noalloc): its job
+// records, tile table, per-pass partials, job sums and counters live in this
+// array instead of buffers the library allocates. Each device that loads the
+// library holds its own copy until the process ends. In host code the name
+// `scratch` is only a stand-in for the symbol, not a device address: the
+// address is looked up once per device (scratch_init, with that device
+// current) and only that address is handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchMutex;
+void* g_scratchPtr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t scratch_init(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (g_scratchPtr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratchPtr[device] = p;
+    return hipSuccess;
+}
+
+void* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    return g_scratchPtr[device];
+}
 
 // Size in bytes of one per-pass partial for (precision mode).
 size_t partial_bytes(bool fp64, int mode) {
