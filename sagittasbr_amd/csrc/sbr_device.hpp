@@ -18,7 +18,7 @@ constexpr int kWideRecBytes = (kWide * 28 + 63) / 64 * 64;  // 6 box floats + 1 
 constexpr int kMaxStack4 = 96;
 constexpr int kMaxDepth4 = kMaxStack4 / (kWide - 1);
 constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a wide record
-constexpr int kCullBoxes = 64;          // boxes of the per-tile emptiness test (one per lane)
+constexpr int kCullBoxes = 256;         // boxes of the tile and ray-patch emptiness tests
 constexpr int kChunkTiles = 64;         // tiles per entry of the tile -> job table
 // Fixed device scratch of the allocation-free sweep (sbr_sweep_noalloc)
 constexpr size_t kScratchBytes = size_t(64) << 20;

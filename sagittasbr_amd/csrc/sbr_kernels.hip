@@ -905,36 +905,49 @@ __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J,
     return st.P;
 }
 
-// Whether some ray of the ray rectangle [ia, ib] x [ja, jb] of a job can
-// reach one of the mesh's cull boxes, which together hold every triangle
-// (lane q tests box q; the result is wave-uniform). A box is reached only by
-// rays whose launch-plane coordinates fall in its projection along the ray
-// direction; the test compares that projection's bounding rectangle, widened
-// by half a ray spacing, 1e-5 of the grid and the host's bound on its FP32
-// rounding (which grows with the mesh's distance from the origin in grid
-// sizes), with the rectangle of ray centres. A tile none of whose rays can reach a
-// box contributes exactly zero and is not traced.
-template <typename R>
-__device__ __forceinline__ bool tile_reaches_mesh(const Job<R>& J, const Mesh<R>& M, int ia, int ib, int ja, int jb,
-                                                  int lane) {
-    bool reach = false;
-    if (lane < M.nCull) {
-        gptr<float> c = as_global<float>(M.cull) + 6 * lane;
-        float amin = -J.cullA[3], amax = -J.cullA[3], bmin = -J.cullB[3], bmax = -J.cullB[3];
+// Rectangle of launch-plane coordinates (a, b) that the projection of one of
+// the mesh's cull boxes along the ray direction covers. The boxes together
+// hold every triangle, so a ray whose (a, b) lies outside every box's
+// rectangle hits nothing. Boxes past the mesh's count give an empty
+// rectangle.
+struct CullRect {
+    float amin, amax, bmin, bmax;
+};
+template <typename R> __device__ __forceinline__ CullRect cull_rect_of(const Job<R>& J, const Mesh<R>& M, int box) {
+    if (box >= M.nCull) return CullRect{3.0e38f, -3.0e38f, 3.0e38f, -3.0e38f};
+    gptr<float> c = as_global<float>(M.cull) + 6 * box;
+    float amin = -J.cullA[3], amax = -J.cullA[3], bmin = -J.cullB[3], bmax = -J.cullB[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const float lo = c[k], hi = c[3 + k];
-            const float pa = lo * J.cullA[k], qa = hi * J.cullA[k];
-            const float pb = lo * J.cullB[k], qb = hi * J.cullB[k];
-            amin += fminf(pa, qa);
-            amax += fmaxf(pa, qa);
-            bmin += fminf(pb, qb);
-            bmax += fmaxf(pb, qb);
-        }
-        const float ma = 1e-5f + 0.5f / float(J.nx) + J.cullErrA, mb = 1e-5f + 0.5f / float(J.ny) + J.cullErrB;
-        const float aLo = (float(ia) + 0.5f) / float(J.nx), aHi = (float(ib) + 0.5f) / float(J.nx);
-        const float bLo = (float(ja) + 0.5f) / float(J.ny), bHi = (float(jb) + 0.5f) / float(J.ny);
-        reach = amax >= aLo - ma && amin <= aHi + ma && bmax >= bLo - mb && bmin <= bHi + mb;
+    for (int k = 0; k < 3; ++k) {
+        const float lo = c[k], hi = c[3 + k];
+        const float pa = lo * J.cullA[k], qa = hi * J.cullA[k];
+        const float pb = lo * J.cullB[k], qb = hi * J.cullB[k];
+        amin += fminf(pa, qa);
+        amax += fmaxf(pa, qa);
+        bmin += fminf(pb, qb);
+        bmax += fmaxf(pb, qb);
+    }
+    return CullRect{amin, amax, bmin, bmax};
+}
+
+// Whether some ray of the ray rectangle [ia, ib] x [ja, jb] of a job can
+// reach one of the kCullBoxes rectangles in rects (lane q tests rectangles
+// q, q + 64, ...; the result is wave-uniform). The rectangles are widened
+// by half a ray spacing, 1e-5 of the grid and the host's bound on their FP32
+// rounding (which grows with the mesh's distance from the origin in grid
+// sizes) and compared with the rectangle of ray centres. Rays of a rectangle
+// that reaches none contribute exactly zero and are not traced.
+template <typename R>
+__device__ __forceinline__ bool rays_reach_mesh(const Job<R>& J, const CullRect* rects, int ia, int ib, int ja, int jb,
+                                                int lane) {
+    const float ma = 1e-5f + 0.5f / float(J.nx) + J.cullErrA, mb = 1e-5f + 0.5f / float(J.ny) + J.cullErrB;
+    const float aLo = (float(ia) + 0.5f) / float(J.nx) - ma, aHi = (float(ib) + 0.5f) / float(J.nx) + ma;
+    const float bLo = (float(ja) + 0.5f) / float(J.ny) - mb, bHi = (float(jb) + 0.5f) / float(J.ny) + mb;
+    bool reach = false;
+#pragma unroll
+    for (int q = lane; q < kCullBoxes; q += 64) {
+        const CullRect r = rects[q];
+        reach = reach || (r.amax >= aLo && r.amin <= aHi && r.bmax >= bLo && r.bmin <= bHi);
     }
     return __ballot(reach) != 0ull;
 }
@@ -986,6 +999,9 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     __shared__ ST sRes[POOL ? 2 * kPoolRays : 1];
     __shared__ int64_t sNext;
     __shared__ unsigned sPool;
+    // projected cull boxes of the current tile's job (one per thread)
+    __shared__ CullRect sCull[(REC || CNT) ? 1 : kCullBoxes];
+    static_assert(kCullBoxes == kWgSize, "one cull box per thread");
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -1034,12 +1050,16 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
         const int tl = static_cast<int>(tile - J.tileBase);
         const int ts = a.tileSide;
         const int tx = tl % J.tilesX, ty = tl / J.tilesX;
-        // tiles none of whose rays can reach the mesh are not traced (not in
-        // record or counting runs, which account for every ray)
+        // tiles and 8x8 ray patches none of whose rays can reach the mesh
+        // are not traced (not in record or counting runs, which account for
+        // every ray). Every wave has passed the previous tile's last barrier,
+        // so the rectangles of its job can be overwritten.
         bool skip = false;
         if constexpr (!REC && !CNT) {
+            sCull[tid] = cull_rect_of(J, M, tid);
+            __syncthreads();
             const int ia = J.i0 + tx * ts, ja = J.j0 + ty * ts;
-            skip = !tile_reaches_mesh(J, M, ia, min(ia + ts - 1, J.nx - 1), ja, min(ja + ts - 1, J.ny - 1), lane);
+            skip = !rays_reach_mesh(J, sCull, ia, min(ia + ts - 1, J.nx - 1), ja, min(ja + ts - 1, J.ny - 1), lane);
         }
 
         using AccT = Acc<R, MODE>;
@@ -1150,8 +1170,13 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 AccT acc = AccT::zero();
                 const int i = J.i0 + tx * ts + px + qx;
                 const int j = J.j0 + ty * ts + py + qy;
+                bool reach = true;
+                if constexpr (!REC && !CNT) {
+                    const int pi = J.i0 + tx * ts + px + (wave & 1) * 8, pj = J.j0 + ty * ts + py + (wave >> 1) * 8;
+                    reach = rays_reach_mesh(J, sCull, pi, min(pi + 7, J.nx - 1), pj, min(pj + 7, J.ny - 1), lane);
+                }
                 do {
-                    if (i >= J.nx || j >= J.ny) break;
+                    if (!reach || i >= J.nx || j >= J.ny) break;
                     if (CNT) cRays++;
                     const int ray = j * J.nx + i;
                     int32_t* ids = REC ? a.recIds + J.recIdBase + size_t(ray) * J.maxB : nullptr;
