@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SBR_ABI_VERSION 2
+#define SBR_ABI_VERSION 3
 
 typedef struct sbr_ctx sbr_ctx;
 typedef struct sbr_plan sbr_plan;
@@ -93,6 +93,8 @@ typedef struct {
     int traversal;   /* loop shape of the BVH traversal: 1 = if-if (a wavefront runs node
                         steps and leaf tests side by side), 2 = while-while (leaves are
                         tested once every lane has reached one; default) */
+    int top_levels;  /* SBR_MIXED, one ray per lane: levels of the four-wide tree each
+                        workgroup stages in LDS (1..4; 0 = none, default) */
 } sbr_tuning;
 
 /* Mesh / BVH statistics. */
@@ -203,8 +205,10 @@ int sbr_plan_destroy(sbr_plan* plan);
  * 16 x 16 threads per block, src/app/sweep.cu:385-398). Times the plan's
  * runs over ray pool {on, off} x LDS stack {8, 16, 32} x tile side
  * {16, 32, 64} x persistent workgroups per CU {occupancy, 2} (batched), or
- * x tile side {16, 32} x hipGraph replay {0, 1} (per-angle), keeps the
- * fastest and applies it to the plan. sample_jobs > 0
+ * x tile side {16, 32} x hipGraph replay {0, 1} (per-angle); then the other
+ * traversal loop shape at the best of those; then, for SBR_MIXED, the top 3
+ * or 4 tree levels staged in LDS at that shape and with an 8-entry LDS
+ * stack. Keeps the fastest and applies it to the plan. sample_jobs > 0
  * tunes on an evenly strided subset of that many jobs (for large sweeps);
  * repeats >= 1 timed runs per candidate (the minimum wall time counts).
  * Results do not depend on the tuning: partial sums are kept per 16 x 16 ray

@@ -93,7 +93,7 @@ class Job(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("wg_per_cu", C.c_int), ("lds_stack", C.c_int), ("tile_rays", C.c_int), ("use_graph", C.c_int),
-                ("ray_pool", C.c_int), ("traversal", C.c_int)]
+                ("ray_pool", C.c_int), ("traversal", C.c_int), ("top_levels", C.c_int)]
 
 
 class MeshInfo(C.Structure):
@@ -288,8 +288,8 @@ class Plan:
         self.h = _P()
         _check(lib().sbr_plan_create(ctx.h, C.cast(jobs, _P), self.n, precision, method, C.byref(self.h)), ctx.h)
 
-    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0, ray_pool=0, traversal=0):
-        t = Tuning(wg_per_cu, lds_stack, tile_rays, use_graph, ray_pool, traversal)
+    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0, ray_pool=0, traversal=0, top_levels=0):
+        t = Tuning(wg_per_cu, lds_stack, tile_rays, use_graph, ray_pool, traversal, top_levels)
         _check(lib().sbr_plan_set_tuning(self.h, C.byref(t)), self.ctx.h)
 
     def autotune(self, sample_jobs=0, repeats=1):
@@ -299,7 +299,8 @@ class Plan:
         _check(lib().sbr_plan_autotune(self.h, sample_jobs, repeats, C.byref(t), C.byref(ms), C.byref(n)),
                self.ctx.h)
         return {"wg_per_cu": t.wg_per_cu, "lds_stack": t.lds_stack, "tile_rays": t.tile_rays,
-                "use_graph": t.use_graph, "ray_pool": t.ray_pool, "traversal": t.traversal, "best_ms": ms.value,
+                "use_graph": t.use_graph, "ray_pool": t.ray_pool, "traversal": t.traversal,
+                "top_levels": t.top_levels, "best_ms": ms.value,
                 "candidates": n.value}
 
     def run(self):

@@ -429,10 +429,45 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
         std::memcpy(&r[6 * W], refs, sizeof(refs));
         recs[it.rec] = r;
     }
-    out.nodes.resize(recs.size() * kWideRecBytes);
-    std::memcpy(out.nodes.data(), recs.data(), out.nodes.size());
-    out.rootRef = 0;
-    out.nInternal = static_cast<int>(recs.size());
+    // the top kTopLevels levels first, breadth-first (so a prefix of the
+    // records is the top k levels, for staging in LDS); every deeper record
+    // keeps its place relative to the others (subtrees stay together)
+    const int nr = static_cast<int>(recs.size());
+    std::vector<int> order, level{0};
+    std::vector<int> newIdx(nr, -1);
+    out.topRecs[0] = 0;
+    for (int d = 0; d < kTopLevels; ++d) {
+        std::vector<int> next;
+        for (int r : level) {
+            newIdx[r] = static_cast<int>(order.size());
+            order.push_back(r);
+            for (int q = 0; q < W; ++q) {
+                int ref;
+                std::memcpy(&ref, &recs[r][6 * W + q], sizeof(int));
+                if (ref >= 0 && ref != kEmptySlot) next.push_back(ref);
+            }
+        }
+        out.topRecs[d + 1] = static_cast<int>(order.size());
+        level.swap(next);
+    }
+    for (int r = 0; r < nr; ++r)
+        if (newIdx[r] < 0) {
+            newIdx[r] = static_cast<int>(order.size());
+            order.push_back(r);
+        }
+    out.nodes.resize(size_t(nr) * kWideRecBytes);
+    for (int k = 0; k < nr; ++k) {
+        std::array<float, F> r = recs[order[k]];
+        for (int q = 0; q < W; ++q) {
+            int ref;
+            std::memcpy(&ref, &r[6 * W + q], sizeof(int));
+            if (ref >= 0 && ref != kEmptySlot) ref = newIdx[ref];
+            std::memcpy(&r[6 * W + q], &ref, sizeof(int));
+        }
+        std::memcpy(out.nodes.data() + size_t(k) * kWideRecBytes, r.data(), kWideRecBytes);
+    }
+    out.rootRef = newIdx[0];
+    out.nInternal = nr;
     out.depth = maxDepth;
     return true;
 }

@@ -71,6 +71,9 @@ struct GpuBvh4 {
     int rootRef = 0;
     int nInternal = 0;
     int depth = 0;  // records on the longest root-to-leaf path
+    // the records of the top kTopLevels levels come first, in breadth-first
+    // order: topRecs[k] records hold levels 0 .. k-1
+    int topRecs[kTopLevels + 1] = {};
 };
 bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err);
 

@@ -26,14 +26,14 @@
 namespace sbr {
 template <typename R>
 hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool, bool ww,
-                        int grid, hipStream_t s);
+                        bool top, int grid, hipStream_t s);
 hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s);
 template <typename R>
 hipError_t launch_finalize(const Job<R>* jobs, int nJobs, int mode, const void* partials, double* jobOut,
                            hipStream_t s);
 size_t partial_bytes(bool fp64, int mode);
 hipError_t launch_adjoint(const AdjArgs& a, int blocks, hipStream_t s);
-template <typename R> int trace_occupancy(int mode, int sd, bool pool);
+template <typename R> int trace_occupancy(int mode, int sd, bool pool, bool top);
 hipError_t scratch_init(int device);
 void* scratch_ptr(int device);
 }  // namespace sbr
@@ -141,7 +141,7 @@ struct sbr_plan {
     DevBuf dChunkJob;  // job of the first tile of each kChunkTiles chunk
     int64_t nChunks = 0;
     int maxDepth = 0;
-    sbr_tuning tune{0, 0, 0, 0, 0, 0};
+    sbr_tuning tune{0, 0, 0, 0, 0, 0, 0};
     int tileSide = 32;
     DevBuf dJobs, dPartials, dJobOut, dCounter, dCounters;
     // mixed mode
@@ -290,6 +290,7 @@ template <typename R> int upload_table(sbr_ctx* ctx) {
             mm.nCull = pm.nCull;
             if constexpr (std::is_same_v<R, float>) {
                 mm.nodes4 = pm.nodes4.p;
+                for (int k = 0; k <= kTopLevels; ++k) mm.topRecs4[k] = pm.gpu4.topRecs[k];
                 mm.rootRef4 = pm.gpu4.rootRef;
                 mm.vertsExact = pm.vertsExact ? 1 : 0;
                 for (int k = 0; k < 3; ++k) {
@@ -726,6 +727,9 @@ template <typename R> int plan_enqueue(sbr_plan* pl, const DevPlan& d, bool reco
     a.recReIm = rReIm;
     a.counters = d.counters;
     const bool mixed = pl->precision == SBR_MIXED;
+    // top levels of the four-wide tree staged in LDS: mixed timed runs, one ray per lane
+    const bool top = mixed && !record && !count && !pool && pl->tune.top_levels > 0;
+    a.topLevels = top ? pl->tune.top_levels : 0;
     if (mixed) {
         a.jobs64 = static_cast<const Job<double>*>(d.jobs64);
         a.meshes64 = static_cast<const Mesh<double>*>(ctx->meshTable64.p);
@@ -744,20 +748,20 @@ template <typename R> int plan_enqueue(sbr_plan* pl, const DevPlan& d, bool reco
             if (pl->totalTiles > INT32_MAX) return fail(&ctx->err, "record run too large for one launch");
             a.tileOffset = 0;
             HIP_TRY(&ctx->err,
-                    launch_trace<R>(a, mode, sd, true, count, false, pool, ww, int(pl->totalTiles), s));
+                    launch_trace<R>(a, mode, sd, true, count, false, pool, ww, false, int(pl->totalTiles), s));
         } else if (pl->method == SBR_BATCHED) {
-            int occ = trace_occupancy<R>(mode, sd, pool);
+            int occ = trace_occupancy<R>(mode, sd, pool, top);
             int per = pl->tune.wg_per_cu > 0 ? pl->tune.wg_per_cu : std::max(occ, 1);
             int64_t grid = int64_t(ctx->cuCount) * per;
             grid = std::min<int64_t>(grid, pl->totalTiles);
             a.tileOffset = 0;
-            HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, false, count, true, pool, ww, int(grid), s));
+            HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, false, count, true, pool, ww, top, int(grid), s));
         } else {
             for (int i = 0; i < pl->nJobs; ++i) {
                 if (pl->nTiles[i] == 0) continue;
                 a.tileOffset = pl->tileBase[i];
                 HIP_TRY(&ctx->err,
-                        launch_trace<R>(a, mode, sd, record, count, false, pool, ww, pl->nTiles[i], s));
+                        launch_trace<R>(a, mode, sd, record, count, false, pool, ww, top, pl->nTiles[i], s));
             }
         }
     }
@@ -1257,6 +1261,8 @@ int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
     if (t->lds_stack < 0 || t->wg_per_cu < 0) return fail(&pl->ctx->err, "negative tuning value");
     if (t->ray_pool < 0 || t->ray_pool > 2) return fail(&pl->ctx->err, "ray_pool must be 0, 1 or 2");
     if (t->traversal < 0 || t->traversal > 2) return fail(&pl->ctx->err, "traversal must be 0, 1 or 2");
+    if (t->top_levels < 0 || t->top_levels > kTopLevels)
+        return fail(&pl->ctx->err, "top_levels must be 0 .. " + std::to_string(kTopLevels));
     if (t->tile_rays != 0 && t->tile_rays != 16 && t->tile_rays != 32 && t->tile_rays != 64)
         return fail(&pl->ctx->err, "tile_rays must be 16, 32 or 64");
     drop_graph(pl);
@@ -1351,10 +1357,10 @@ int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* be
         for (int sd : {8, 16, 32}) {
             if (pl->method == SBR_BATCHED) {
                 for (int ts : {16, 32, 64})
-                    for (int wg : {0, 2}) cands.push_back(sbr_tuning{wg, sd, ts, 0, pool, 0});
+                    for (int wg : {0, 2}) cands.push_back(sbr_tuning{wg, sd, ts, 0, pool, 0, 0});
             } else {
                 for (int ts : {16, 32})
-                    for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool, 0});
+                    for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool, 0, 0});
             }
         }
     double bestT = 1e300;
@@ -1386,6 +1392,21 @@ int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* be
         sbr_tuning c = bestC;
         c.traversal = default_traversal() == 2 ? 1 : 2;
         rc = trial(c);
+    }
+    // stage 3 (mixed, one ray per lane): the tree's top levels in LDS, at
+    // the best shape and with the smallest LDS stack (the staged records
+    // and a deeper stack together leave room for fewer workgroups per CU)
+    if (!rc && pl->precision == SBR_MIXED && bestC.ray_pool != 1) {
+        const sbr_tuning base = bestC;
+        std::vector<int> sds{base.lds_stack > 0 ? base.lds_stack : 8};
+        if (sds[0] != 8) sds.push_back(8);
+        for (int sd : sds)
+            for (int k : {3, 4}) {
+                sbr_tuning c = base;
+                c.lds_stack = sd;
+                c.top_levels = k;
+                if (!rc) rc = trial(c);
+            }
     }
     if (owned) sbr_plan_destroy(owned);
     if (rc) return rc;

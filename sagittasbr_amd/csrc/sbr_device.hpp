@@ -19,6 +19,10 @@ constexpr int kMaxStack4 = 96;
 constexpr int kMaxDepth4 = kMaxStack4 / (kWide - 1);
 constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a wide record
 constexpr int kCullBoxes = 256;         // boxes of the tile and ray-patch emptiness tests
+// Top levels of the four-wide tree that a workgroup can stage in LDS: at
+// most 1 + 4 + 16 + 64 = 85 records of kWideRecBytes
+constexpr int kTopLevels = 4;
+constexpr int kTopRecs = 85;
 constexpr int kChunkTiles = 64;         // tiles per entry of the tile -> job table
 // Fixed device scratch of the allocation-free sweep (sbr_sweep_noalloc)
 constexpr size_t kScratchBytes = size_t(64) << 20;
@@ -66,6 +70,9 @@ template <typename R> struct alignas(16) Mesh {
     int vertsExact;
     // FP32 tree only: the four-wide collapse of the tree (mixed mode)
     const void* nodes4;
+    // FP32 tree only: records of the four-wide tree's top k levels, which
+    // come first (topRecs4[k], k = 0 .. kTopLevels)
+    int topRecs4[kTopLevels + 1];
     // up to kCullBoxes boxes (lo.xyz hi.xyz, FP32, rounded outwards) that
     // together hold every triangle, for the per-tile emptiness test
     const float* cull;
@@ -112,6 +119,8 @@ template <typename R> struct TraceArgs {
     const Mesh<double>* meshes64;
     float errK;
     int8_t* recFlag;
+    // levels of the four-wide tree staged in LDS (mixed mode, 1 .. kTopLevels)
+    int topLevels;
 };
 
 // Arguments of the frozen-path adjoint kernel (one launch per job).

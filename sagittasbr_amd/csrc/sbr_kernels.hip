@@ -221,6 +221,22 @@ __device__ __forceinline__ NodeW load_nodew(const void* nodes, int i) {
     return n;
 }
 
+// The same record from the workgroup's LDS copy of the tree's top levels
+// (records of kWideRecBytes, 16-byte aligned).
+__device__ __forceinline__ NodeW load_nodew_lds(const vf4* top, int i) {
+    const vf4* p = top + (kWideRecBytes / 16) * i;
+    NodeW n;
+#pragma unroll
+    for (int k = 0; k < 7 * kWide / 4; ++k) {
+        const vf4 v = p[k];
+        n.f[4 * k] = v.x;
+        n.f[4 * k + 1] = v.y;
+        n.f[4 * k + 2] = v.z;
+        n.f[4 * k + 3] = v.w;
+    }
+    return n;
+}
+
 // Sorting network on (key, value) pairs, ascending keys.
 template <int W> __device__ __forceinline__ void sort_net(float* e, int* r) {
     // compare-exchange as selects (a branch costs exec-mask updates and moves)
@@ -434,12 +450,12 @@ constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
 // along the ray by shift (first query): FP32 box distances are then shift
 // shorter than the FP64 ones. Unless amb is set, a hit's FP64 vertices and
 // face id are returned in hv0, hv1, hv2 and hFace.
-template <int SD, bool CNT, bool WW>
+template <int SD, bool CNT, bool WW, bool TOP>
 __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> inv, V3<double> o64,
                                                  V3<double> d64, double shift, double& tHit, int* sRef, float* sE,
-                                                 Ov<float>::T* ov, int tid, Work& w, float ex, float ed, float K,
-                                                 bool& amb, V3<double>& hv0, V3<double>& hv1, V3<double>& hv2,
-                                                 int& hFace) {
+                                                 Ov<float>::T* ov, const vf4* top, int nTop, int tid, Work& w,
+                                                 float ex, float ed, float K, bool& amb, V3<double>& hv0,
+                                                 V3<double>& hv1, V3<double>& hv2, int& hFace) {
     const float o1 = fabsf(o.x) + fabsf(o.y) + fabsf(o.z);
     // far distance of anything in the root box from the origin
     float cx = 0.5f * (m.rootLo4[0] + m.rootHi4[0]), cy = 0.5f * (m.rootLo4[1] + m.rootHi4[1]),
@@ -477,7 +493,8 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     // does not depend on the order)
     auto node_step = [&]() -> bool {
         constexpr int W = kWide;
-        const NodeW n = load_nodew(m.nodes4, cur);
+        // records below nTop are the top levels the workgroup staged in LDS
+        const NodeW n = (TOP && cur < nTop) ? load_nodew_lds(top, cur) : load_nodew(m.nodes4, cur);
         if (CNT) w.visits += W;
         float e[W];
         int r[W];
@@ -824,10 +841,11 @@ __device__ __forceinline__ void ray_init_mixed(const Job<double>& D, int i, int 
 // (a near-tie or a hit on the edge of its box) is answered by the reference
 // FP64 query on the FP64 tree from the same FP64 ray state -- the reference
 // query itself -- and sets fell.
-template <int SD, bool CNT, bool REC, bool WW>
+template <int SD, bool CNT, bool REC, bool WW, bool TOP>
 __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<float>& M, const Mesh<double>& M64,
                                                RayState<double>& st, int* sRef, float* sE, Ov<float>::T* ov,
-                                               Ov<double>::T* ov64, int tid, int32_t* ids, Work& w,
+                                               Ov<double>::T* ov64, const vf4* top, int nTop, int tid, int32_t* ids,
+                                               Work& w,
                                                unsigned long long& queries, unsigned long long& fullQ, float K,
                                                bool& fell) {
     const V3<double> o = st.o, d = st.d;
@@ -850,8 +868,8 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
     bool amb = false;
     V3<double> v0, v1, v2, e1, e2;
     int face;
-    int h = closest_hit_mixed<SD, CNT, WW>(M, of, inv, o, d, shift, t, sRef, sE, ov, tid, w, ex, ed, K, amb, v0, v1,
-                                           v2, face);
+    int h = closest_hit_mixed<SD, CNT, WW, TOP>(M, of, inv, o, d, shift, t, sRef, sE, ov, top, nTop, tid, w, ex, ed,
+                                                K, amb, v0, v1, v2, face);
     if (CNT) queries++;
     // FP64 edges of the chosen triangle (rtKernels.cu:13-14)
     if (!amb) {
@@ -888,18 +906,19 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
 
 // Whole mixed-mode path in one call; fell is set when a query was answered
 // on the FP64 tree.
-template <int SD, bool CNT, bool REC, bool WW>
+template <int SD, bool CNT, bool REC, bool WW, bool TOP>
 __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J, const Job<double>& D,
                                                             const Mesh<float>& M, const Mesh<double>& M64, int i,
                                                             int j, int* sRef, float* sE, Ov<float>::T* ov,
-                                                            Ov<double>::T* ov64, int tid, int32_t* ids, Work& w,
+                                                            Ov<double>::T* ov64, const vf4* top, int nTop, int tid,
+                                                            int32_t* ids, Work& w,
                                                             unsigned long long& queries, unsigned long long& fullQ,
                                                             float K, bool& fell) {
     RayState<double> st;
     ray_init_mixed(D, i, j, st);
     if (J.maxB > 0)
-        while (!ray_step_mixed<SD, CNT, REC, WW>(J, M, M64, st, sRef, sE, ov, ov64, tid, ids, w, queries, fullQ, K,
-                                                 fell)) {
+        while (!ray_step_mixed<SD, CNT, REC, WW, TOP>(J, M, M64, st, sRef, sE, ov, ov64, top, nTop, tid, ids, w,
+                                                      queries, fullQ, K, fell)) {
         }
     st.P.Ld = st.P.L;
     return st.P;
@@ -986,9 +1005,10 @@ constexpr int kPoolRays = 1024;  // rays per pool batch (4 passes of 16x16)
 template <typename R, int MODE> constexpr int kWavesPerEu =
     MODE == 1 ? SBR_WPE_MIXED : (std::is_same_v<R, double> ? SBR_WPE_FP64 : SBR_WPE_FP32);
 
-template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW>
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW, bool TOP = false>
 __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<R, MODE>))) void trace_kernel(
     TraceArgs<R> a) {
+    static_assert(!TOP || (MODE == 1 && !REC && !CNT && !POOL), "top-level staging: mixed mode, timed runs");
     using ST = typename SlotOf<R, MODE>::T;
     using RS = typename std::conditional<MODE == 1, double, R>::type;  // ray state precision
     __shared__ int sRef[SD * kWgSize];
@@ -1001,6 +1021,9 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     __shared__ unsigned sPool;
     // projected cull boxes of the current tile's job (one per thread)
     __shared__ CullRect sCull[(REC || CNT) ? 1 : kCullBoxes];
+    // the top levels of the current mesh's four-wide tree (TOP)
+    __shared__ vf4 sTop[TOP ? kTopRecs * (kWideRecBytes / 16) : 1];
+    int stagedMesh = -1, nTop = 0;
     static_assert(kCullBoxes == kWgSize, "one cull box per thread");
 
     const int tid = threadIdx.x;
@@ -1055,6 +1078,16 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
         // every ray). Every wave has passed the previous tile's last barrier,
         // so the rectangles of its job can be overwritten.
         bool skip = false;
+        if constexpr (TOP) {
+            // copy the mesh's top levels once per change of mesh (made visible
+            // by the barrier below)
+            if (J.mesh != stagedMesh) {
+                stagedMesh = J.mesh;
+                nTop = min(M.topRecs4[a.topLevels], kTopRecs);
+                const gptr<vf4> src = as_global<vf4>(M.nodes4);
+                for (int k = tid; k < nTop * (kWideRecBytes / 16); k += kWgSize) sTop[k] = src[k];
+            }
+        }
         if constexpr (!REC && !CNT) {
             sCull[tid] = cull_rect_of(J, M, tid);
             __syncthreads();
@@ -1114,9 +1147,9 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     if (alive) {
                         bool done;
                         if constexpr (MODE == 1) {
-                            done = ray_step_mixed<SD, CNT, false, WW>(
-                                J, M, a.meshes64[J.mesh], st, sRef, sE, ov, ov64, tid, nullptr, w, cQueries, cFull,
-                                a.errK, fell);
+                            done = ray_step_mixed<SD, CNT, false, WW, false>(
+                                J, M, a.meshes64[J.mesh], st, sRef, sE, ov, ov64, nullptr, 0, tid, nullptr, w,
+                                cQueries, cFull, a.errK, fell);
                         } else {
                             done = ray_step<R, SD, CNT, false, WW>(J, M, st, sRef, sE, ov, tid, nullptr, w,
                                                                    cQueries);
@@ -1187,9 +1220,9 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     bool fell = false;
                     if constexpr (MODE == 1) {
                         const Job<double>& D = a.jobs64[jid];
-                        const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW>(
-                            J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, tid, ids, w, cQueries, cFull,
-                            a.errK, fell);
+                        const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW, TOP>(
+                            J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, sTop, nTop, tid, ids, w, cQueries,
+                            cFull, a.errK, fell);
                         po_term<double, true>(D, PD, dre, dimg);
                         P.count = PD.count;
                         P.Ld = PD.Ld;
@@ -1446,12 +1479,29 @@ hipError_t launch_one(const TraceArgs<R>& a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW, bool TOP>
+hipError_t launch_top(const TraceArgs<R>& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST, POOL, WW, TOP>), dim3(grid), dim3(kWgSize), 0, s,
+                       a);
+    return hipGetLastError();
+}
+
 // Record and counting runs use the while-while shape only (their outputs
-// do not depend on the shape).
+// do not depend on the shape); top: stage the tree's top levels in LDS
+// (mixed mode, one ray per lane).
 template <typename R, int MODE, int SD, bool POOL>
-hipError_t launch_pool(const TraceArgs<R>& a, bool cnt, bool persist, bool ww, int grid, hipStream_t s) {
+hipError_t launch_pool(const TraceArgs<R>& a, bool cnt, bool persist, bool ww, bool top, int grid, hipStream_t s) {
     if (cnt) return persist ? launch_one<R, MODE, SD, false, true, true, POOL, true>(a, grid, s)
                             : launch_one<R, MODE, SD, false, true, false, POOL, true>(a, grid, s);
+    if constexpr (MODE == 1 && !POOL) {
+        if (top) {
+            if (ww)
+                return persist ? launch_top<R, MODE, SD, false, false, true, POOL, true, true>(a, grid, s)
+                               : launch_top<R, MODE, SD, false, false, false, POOL, true, true>(a, grid, s);
+            return persist ? launch_top<R, MODE, SD, false, false, true, POOL, false, true>(a, grid, s)
+                           : launch_top<R, MODE, SD, false, false, false, POOL, false, true>(a, grid, s);
+        }
+    }
     if (ww)
         return persist ? launch_one<R, MODE, SD, false, false, true, POOL, true>(a, grid, s)
                        : launch_one<R, MODE, SD, false, false, false, POOL, true>(a, grid, s);
@@ -1460,11 +1510,11 @@ hipError_t launch_pool(const TraceArgs<R>& a, bool cnt, bool persist, bool ww, i
 }
 
 template <typename R, int MODE, int SD>
-hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, bool pool, bool ww, int grid,
+hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, bool pool, bool ww, bool top, int grid,
                      hipStream_t s) {
     if (rec) return launch_one<R, MODE, SD, true, false, false, false, true>(a, grid, s);
-    return pool ? launch_pool<R, MODE, SD, true>(a, cnt, persist, ww, grid, s)
-                : launch_pool<R, MODE, SD, false>(a, cnt, persist, ww, grid, s);
+    return pool ? launch_pool<R, MODE, SD, true>(a, cnt, persist, ww, false, grid, s)
+                : launch_pool<R, MODE, SD, false>(a, cnt, persist, ww, top, grid, s);
 }
 
 }  // namespace
@@ -1474,22 +1524,22 @@ hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, bo
 // traversal loop (else if-if).
 template <typename R>
 hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool, bool ww,
-                        int grid, hipStream_t s) {
+                        bool top, int grid, hipStream_t s) {
     if constexpr (std::is_same_v<R, float>) {
         if (mode == 1) {
-            if (sd <= 8) return launch_sd<float, 1, 8>(a, rec, cnt, persist, pool, ww, grid, s);
-            if (sd <= 16) return launch_sd<float, 1, 16>(a, rec, cnt, persist, pool, ww, grid, s);
-            return launch_sd<float, 1, 32>(a, rec, cnt, persist, pool, ww, grid, s);
+            if (sd <= 8) return launch_sd<float, 1, 8>(a, rec, cnt, persist, pool, ww, top, grid, s);
+            if (sd <= 16) return launch_sd<float, 1, 16>(a, rec, cnt, persist, pool, ww, top, grid, s);
+            return launch_sd<float, 1, 32>(a, rec, cnt, persist, pool, ww, top, grid, s);
         }
     }
-    if (sd <= 8) return launch_sd<R, 0, 8>(a, rec, cnt, persist, pool, ww, grid, s);
-    if (sd <= 16) return launch_sd<R, 0, 16>(a, rec, cnt, persist, pool, ww, grid, s);
-    return launch_sd<R, 0, 32>(a, rec, cnt, persist, pool, ww, grid, s);
+    if (sd <= 8) return launch_sd<R, 0, 8>(a, rec, cnt, persist, pool, ww, false, grid, s);
+    if (sd <= 16) return launch_sd<R, 0, 16>(a, rec, cnt, persist, pool, ww, false, grid, s);
+    return launch_sd<R, 0, 32>(a, rec, cnt, persist, pool, ww, false, grid, s);
 }
 
-template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool, bool, bool, bool, bool, int,
+template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool, bool, bool, bool, bool, bool, int,
                                         hipStream_t);
-template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, bool, bool, int,
+template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, bool, bool, bool, int,
                                          hipStream_t);
 
 template <typename R>
@@ -1522,25 +1572,30 @@ hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <typename R, int MODE, int SD> int occ_one(bool pool) {
+template <typename R, int MODE, int SD> int occ_one(bool pool, bool top) {
     int n = 0;
     const void* f = pool ? reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, true, true>)
                          : reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, false, true>);
+    if constexpr (MODE == 1) {
+        if (top && !pool)
+            f = reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, false, true, true>);
+    }
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, kWgSize, 0);
     return e == hipSuccess ? n : 1;
 }
 
 // Resident workgroups per CU of the persistent kernel variant.
-template <typename R> int trace_occupancy(int mode, int sd, bool pool) {
+template <typename R> int trace_occupancy(int mode, int sd, bool pool, bool top) {
     if constexpr (std::is_same_v<R, float>) {
         if (mode == 1)
-            return sd <= 8 ? occ_one<float, 1, 8>(pool)
-                           : (sd <= 16 ? occ_one<float, 1, 16>(pool) : occ_one<float, 1, 32>(pool));
+            return sd <= 8 ? occ_one<float, 1, 8>(pool, top)
+                           : (sd <= 16 ? occ_one<float, 1, 16>(pool, top) : occ_one<float, 1, 32>(pool, top));
     }
-    return sd <= 8 ? occ_one<R, 0, 8>(pool) : (sd <= 16 ? occ_one<R, 0, 16>(pool) : occ_one<R, 0, 32>(pool));
+    return sd <= 8 ? occ_one<R, 0, 8>(pool, false)
+                   : (sd <= 16 ? occ_one<R, 0, 16>(pool, false) : occ_one<R, 0, 32>(pool, false));
 }
-template int trace_occupancy<float>(int, int, bool);
-template int trace_occupancy<double>(int, int, bool);
+template int trace_occupancy<float>(int, int, bool, bool);
+template int trace_occupancy<double>(int, int, bool, bool);
 
 // ------------------------------------------------ fixed scratch memory
 // Device storage of the allocation-free sweep (sbr_sweep_noalloc): its job

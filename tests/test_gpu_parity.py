@@ -410,6 +410,15 @@ def test_tuning_variants(ctx, tri_duct, prec):
                 pl.wait()
                 z, _, _ = pl.results()
                 assert np.array_equal(z, z0), (trav, pool, ts)
+    # the mixed mode's top tree levels staged in LDS (ignored elsewhere)
+    for top in (1, 3, 4):
+        for trav in (1, 2):
+            _step(f"precision {prec} batched top_levels={top} traversal={trav}")
+            pl.tune(top_levels=top, traversal=trav, tile_rays=32)
+            pl.run()
+            pl.wait()
+            z, _, _ = pl.results()
+            assert np.array_equal(z, z0), (top, trav)
     pl.close()
     pa = ctx.plan(jobs, prec, S.SBR_PER_ANGLE)
     for pool in (1, 2):
