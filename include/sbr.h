@@ -129,6 +129,15 @@ int sbr_ctx_create(int device, sbr_ctx** out);
  * lose their device data too: every later call on them fails with an error,
  * and sbr_plan_destroy still frees their host side. */
 int sbr_ctx_destroy(sbr_ctx* ctx);
+/* Run the context's work on a caller-owned HIP stream (a hipStream_t of the
+ * context's device from the HIP runtime libsbr is linked against, passed as
+ * void*), or back on the context's own stream with NULL. The reference issues its per-angle work on its own two streams
+ * (src/app/sweep.cu:119-133); this lets a caller order sweeps with its own
+ * work instead. The caller keeps the stream alive while the context uses it
+ * and must not destroy it before switching back. Plans captured in a
+ * hipGraph replay on whichever stream is current. */
+int sbr_ctx_set_stream(sbr_ctx* ctx, void* stream);
+
 /* Number of compute units of the context's device. */
 int sbr_ctx_cu_count(const sbr_ctx* ctx, int* out);
 

@@ -149,6 +149,7 @@ def lib():
         "sbr_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
         "sbr_ctx_destroy": (C.c_int, [_P]),
         "sbr_ctx_cu_count": (C.c_int, [_P, C.POINTER(C.c_int)]),
+        "sbr_ctx_set_stream": (C.c_int, [_P, _P]),
         "sbr_mesh_create": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, C.c_double, _P, C.POINTER(C.c_int)]),
         "sbr_mesh_info_get": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(MeshInfo)]),
         "sbr_mesh_permutation": (C.c_int, [_P, C.c_int, C.c_int, _P]),
@@ -350,6 +351,13 @@ class Context:
 
     def __del__(self):
         self.close()
+
+    def set_stream(self, stream=None):
+        """Issue the context's work on a caller's HIP stream (a raw hipStream_t
+        handle from the HIP runtime libsbr.so links, /opt/rocm/lib), or on its
+        own with None. torch's ROCm wheel carries a HIP runtime of its own:
+        its stream handles are not valid here."""
+        _check(lib().sbr_ctx_set_stream(self.h, None if stream is None else C.c_void_p(int(stream))), self.h)
 
     @property
     def cu_count(self):

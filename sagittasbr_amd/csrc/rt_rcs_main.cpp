@@ -331,6 +331,18 @@ int run(int argc, char** argv) {
         return 1;
     }
     double sweepMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
+    // progress lines of every 100th angle and the last (sweep.cu:197,236-303),
+    // after the fact: the ray and PO work of an angle is one fused launch
+    // (and the batched method sweeps every angle in one launch), so the time
+    // shown is the sweep's kernel time per angle
+    for (int i = 0; i < n; ++i) {
+        if ((i + 1) % 100 != 0 && i != n - 1) continue;
+        std::cerr << "[R0:" << std::setw(3) << i + 1 << "/" << n << "]"
+                  << " │ Θ: " << std::setw(3) << static_cast<int>(jobs[i].theta_deg) << "°"
+                  << " │ Φ: " << std::setw(3) << static_cast<int>(jobs[i].phi_deg) << "°"
+                  << " │ RT+PO: " << std::fixed << std::setprecision(2) << kernelMs / n << "ms (sweep mean)"
+                  << " │ σ: " << std::setprecision(2) << dbsm[i] << " dBsm" << std::defaultfloat << "\n";
+    }
     // rows sorted by global index (sweep.cu:529-545)
     for (int i = 0; i < n; ++i)
         out << fmt_real(jobs[i].theta_deg, c.precision) << "," << fmt_real(jobs[i].phi_deg, c.precision) << ","

@@ -116,7 +116,8 @@ struct MeshEntry {
 struct sbr_ctx {
     int device = 0;
     int cuCount = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     // the stream work is issued on
+    hipStream_t ownStream = nullptr;  // the context's own stream
     std::string err;
     std::vector<std::unique_ptr<MeshEntry>> meshes;
     DevBuf meshTable32, meshTable64;
@@ -955,7 +956,8 @@ int sbr_ctx_create(int device, sbr_ctx** out) {
     ctx->device = device;
     HIP_TRY(nullptr, hipSetDevice(device));
     HIP_TRY(nullptr, hipDeviceGetAttribute(&ctx->cuCount, hipDeviceAttributeMultiprocessorCount, device));
-    HIP_TRY(nullptr, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(nullptr, hipStreamCreateWithFlags(&ctx->ownStream, hipStreamNonBlocking));
+    ctx->stream = ctx->ownStream;
     // device address of the allocation-free sweep's scratch array, looked up
     // once per device with the device current
     if (hipError_t e = scratch_init(device); e != hipSuccess)
@@ -972,8 +974,20 @@ int sbr_ctx_destroy(sbr_ctx* ctx) {
     for (sbr_plan* pl : ctx->plans) release_plan_device(pl);
     ctx->plans.clear();
     ctx->meshes.clear();
-    (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream != ctx->ownStream) (void)hipStreamSynchronize(ctx->ownStream);
+    (void)hipStreamDestroy(ctx->ownStream);
     delete ctx;
+    return 0;
+}
+
+int sbr_ctx_set_stream(sbr_ctx* ctx, void* stream) {
+    if (!ctx) return fail(nullptr, "null context");
+    HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
+    hipStream_t next = stream ? static_cast<hipStream_t>(stream) : ctx->ownStream;
+    if (next == ctx->stream) return 0;
+    // work already issued finishes before later work starts on the new stream
+    HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
+    ctx->stream = next;
     return 0;
 }
 

@@ -307,3 +307,40 @@ def test_autograd_function(ctx):
     np.testing.assert_allclose(sigma.detach().numpy(), sg, rtol=1e-12)
     np.testing.assert_allclose(vt.grad.numpy(), gx, rtol=1e-9, atol=1e-12 * np.abs(gx).max())
     assert abs(ft.grad.item() - float(w.numpy() @ df)) <= 1e-9 * abs(float(w.numpy() @ df))
+
+
+@pytest.mark.gpu
+def test_autograd_mesh_cache_and_single_adjoint(ctx):
+    """Forward runs one sweep on a cached mesh; backward runs one adjoint.
+    Re-evaluating a geometry reuses its mesh, at most two stay alive, and a
+    backward pass whose mesh was evicted builds it again with the same
+    gradient."""
+    import torch
+
+    from sagittasbr_amd import adjoint as A
+
+    v, f = M.icosphere(3)
+    th, ph = np.array([90.0, 30.0]), np.array([10.0, 200.0])
+    kw = dict(grid_size=2.5, n_x=80, n_y=80, max_bounces=4)
+    eng = S.Context(0)
+    try:
+        vt = torch.tensor(v.astype(np.float64), requires_grad=True)
+        s1 = A.monostatic_rcs(eng, vt, f, th, ph, 3e9, **kw)
+        s2 = A.monostatic_rcs(eng, vt, f, th, ph, 3e9, **kw)
+        assert len(eng._adjoint_meshes) == 1
+        assert torch.equal(s1, s2)
+        s1.sum().backward()
+        g_first = vt.grad.clone()
+        # two other geometries push the first mesh out of the cache
+        for k in (1, 2):
+            vk = torch.tensor((v * (1 + 0.01 * k)).astype(np.float64), requires_grad=True)
+            A.monostatic_rcs(eng, vk, f, th, ph, 3e9, **kw)
+        assert len(eng._adjoint_meshes) == 2
+        vt.grad = None
+        s2.sum().backward()  # its mesh was evicted: rebuilt for the backward pass
+        # (FP64 atomics: equal to rounding)
+        np.testing.assert_allclose(vt.grad.numpy(), g_first.numpy(), rtol=1e-12,
+                                   atol=1e-13 * np.abs(g_first.numpy()).max())
+        assert len(eng._adjoint_meshes) == 2
+    finally:
+        eng.close()

@@ -311,6 +311,40 @@ def test_graph_replay_matches_eager(ctx, sphere, tri_duct):
         pl.close()
 
 
+def test_caller_stream(ctx, sphere):
+    """Sweeps issued on a caller's stream (eager and hipGraph replay) give the
+    sums of the context's own stream. The stream comes from the HIP runtime
+    libsbr.so is linked against (torch's wheel carries a runtime of its own,
+    whose streams are not valid there)."""
+    import ctypes
+
+    v, f, mesh, s32, s64 = sphere
+    p = S.SweepParams.make(**dict(C1, phi_end=300, phi_samples=6))
+    z0, _, _ = ctx.sweep(mesh, p, S.SBR_MIXED, S.SBR_BATCHED)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    st = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0
+    ctx.set_stream(st.value)
+    try:
+        _step("sweep on the caller's stream")
+        z1, _, _ = ctx.sweep(mesh, p, S.SBR_MIXED, S.SBR_BATCHED)
+        pl = ctx.plan(S.sweep_jobs(p, mesh, S.SBR_MIXED), S.SBR_MIXED, S.SBR_BATCHED)
+        pl.tune(use_graph=1)
+        for _ in range(2):
+            _step("graph replay on the caller's stream")
+            pl.run()
+            pl.wait()
+            z2, _, _ = pl.results()
+            assert np.array_equal(z2, z0)
+        pl.close()
+    finally:
+        ctx.set_stream(None)
+        hip.hipStreamDestroy(st)
+    assert np.array_equal(z1, z0)
+    z3, _, _ = ctx.sweep(mesh, p, S.SBR_MIXED, S.SBR_BATCHED)
+    assert np.array_equal(z3, z0)
+
+
 def test_graph_replay_after_mesh_table_grows():
     """A plan captured in a hipGraph holds the device address of the mesh
     table; meshes created later grow the table to a new address. The next

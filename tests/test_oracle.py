@@ -127,6 +127,57 @@ def test_frame_invariants():
         np.testing.assert_allclose(-d, [np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], atol=1e-12)
 
 
+def _frame_both(theta, phi, g):
+    """The launch frame of one (theta, phi) from the oracle (FP64) and from
+    libsbr in FP32 and FP64."""
+    kw = dict(theta_start=theta, theta_end=theta, theta_samples=1, phi_start=phi, phi_end=phi, phi_samples=1,
+              grid_size=g, n_x=10, n_y=10)
+    out = [O.frame(O.Params(**kw), 0, precision=1)[0]]
+    try:
+        import sagittasbr_amd as S
+    except Exception:  # the library is optional for the oracle's own tests
+        return out
+    for prec in (S.SBR_FP32, S.SBR_FP64):
+        out.append(S.angle_frame(S.SweepParams.make(**kw), 0, prec)[0])
+    return out
+
+
+@pytest.mark.parametrize("theta,phi,U,V", [
+    # sweep.cu:349-364 by hand: dir = (sin t cos p, sin t sin p, cos t); up = +y,
+    # or +x once |dir.y| > 0.99; U = unit(up x dir) g; V = unit(dir x U) g
+    (90.0, 0.0, (0, 0, -1), (0, 1, 0)),
+    (90.0, 90.0, (0, 0, 1), (1, 0, 0)),
+    (90.0, 180.0, (0, 0, 1), (0, 1, 0)),
+    (0.0, 0.0, (1, 0, 0), (0, 1, 0)),
+])
+def test_frame_known_answers(theta, phi, U, V):
+    g = 2.5
+    for llc, u, v, rd in _frame_both(theta, phi, g):
+        d = -np.asarray(rd)
+        np.testing.assert_allclose(u, np.array(U) * g, atol=2e-6 * g)
+        np.testing.assert_allclose(v, np.array(V) * g, atol=2e-6 * g)
+        th, ph = np.radians(theta), np.radians(phi)
+        np.testing.assert_allclose(d, [np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], atol=1e-6)
+        np.testing.assert_allclose(llc, d * 10 * g - 0.5 * u - 0.5 * v, atol=1e-5 * g)
+
+
+@pytest.mark.parametrize("sy", [0.9899, 0.9901])
+def test_frame_up_vector_switch(sy):
+    """Either side of |dir.y| = 0.99 the up vector switches from +y to +x
+    (sweep.cu:353-354), which flips U from -z to +z and V from (-dy, dx, 0) to
+    (dy, -dx, 0) for a direction in the x-y plane."""
+    g = 1.0
+    phi = float(np.degrees(np.arcsin(sy)))
+    for llc, u, v, rd in _frame_both(90.0, phi, g):
+        d = -np.asarray(rd)
+        if sy < 0.99:
+            np.testing.assert_allclose(u, [0, 0, -1], atol=2e-6)
+            np.testing.assert_allclose(v, [-d[1], d[0], 0], atol=2e-6)
+        else:
+            np.testing.assert_allclose(u, [0, 0, 1], atol=2e-6)
+            np.testing.assert_allclose(v, [d[1], -d[0], 0], atol=2e-6)
+
+
 # ------------------------------------------------------- closed forms
 @pytest.mark.parametrize("freq", [3e9, 10e9])
 def test_flat_plate_closed_form(freq):
