@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -116,6 +117,142 @@ float obj_real(const char*& p) {
 
 }  // namespace
 
+namespace {
+
+// One face corner: position, texcoord and normal index (0-based, -1 when
+// absent).
+struct Corner {
+    int v = -1, vt = -1, vn = -1;
+};
+
+// OBJ index -> 0-based (1-based, negative = relative to the count so far;
+// 0 is invalid), as tinyobj's fixIndex does.
+bool fix_index(int idx, int n, int& out) {
+    if (idx > 0) {
+        out = idx - 1;
+        return true;
+    }
+    if (idx == 0) return false;
+    out = n + idx;
+    return true;
+}
+
+// "v", "v/vt", "v//vn" or "v/vt/vn" (tinyobj's parseTriple).
+bool parse_corner(const char* p, const char* end, int nv, int nvt, int nvn, Corner& c) {
+    auto num = [&](const char*& q, int& val) -> bool {
+        const char* s0 = q;
+        val = std::atoi(q);
+        if (*q == '+' || *q == '-') ++q;
+        while (q < end && std::isdigit(static_cast<unsigned char>(*q))) ++q;
+        return q > s0;
+    };
+    const char* q = p;
+    int a = 0;
+    if (!num(q, a) || !fix_index(a, nv, c.v)) return false;
+    if (q < end && *q == '/') {
+        ++q;
+        if (q < end && *q != '/') {
+            int b = 0;
+            if (num(q, b) && !fix_index(b, nvt, c.vt)) return false;
+        }
+        if (q < end && *q == '/') {
+            ++q;
+            int d = 0;
+            if (num(q, d) && !fix_index(d, nvn, c.vn)) return false;
+        }
+    }
+    return true;
+}
+
+// W. R. Franklin's crossing-number point-in-polygon test, in float, as
+// tinyobj applies it to a candidate ear (tinyOBJLoader.hpp:1440-1450).
+bool inside_polygon(int n, const float* x, const float* y, float tx, float ty) {
+    bool c = false;
+    for (int a = 0, b = n - 1; a < n; b = a++)
+        if (((y[a] > ty) != (y[b] > ty)) && (tx < (x[b] - x[a]) * (ty - y[a]) / (y[b] - y[a]) + x[a])) c = !c;
+    return c;
+}
+
+// Triangulation of one face with 5+ corners by tinyobj's built-in ear
+// clipping (tinyOBJLoader.hpp:1735-1948, its default without earcut):
+// project onto the two axes chosen from the first corner with a
+// non-degenerate turn; walk a guess index around the remaining polygon,
+// skipping corners whose turn has the opposite sign to the fan term
+// (x0 y1 - y0 x1) / 2 and corners whose triangle contains another remaining
+// corner; emit each accepted ear and drop its middle corner; stop when the
+// polygon is a triangle (emitted) or no corner was accepted in a full turn.
+void clip_ears(const std::vector<Corner>& face, const std::vector<float>& v, std::vector<Corner>& tris) {
+    const size_t nf = face.size();
+    auto coord = [&](int vi, int axis) -> float {
+        const size_t k = size_t(vi) * 3 + size_t(axis);
+        return vi >= 0 && k < v.size() ? v[k] : 0.0f;
+    };
+    int ax0 = 1, ax1 = 2;
+    for (size_t k = 0; k < nf; ++k) {
+        const int i0 = face[k].v, i1 = face[(k + 1) % nf].v, i2 = face[(k + 2) % nf].v;
+        if (i0 < 0 || i1 < 0 || i2 < 0 || size_t(i0) * 3 + 2 >= v.size() || size_t(i1) * 3 + 2 >= v.size() ||
+            size_t(i2) * 3 + 2 >= v.size())
+            continue;
+        const float* p0 = &v[size_t(i0) * 3];
+        const float* p1 = &v[size_t(i1) * 3];
+        const float* p2 = &v[size_t(i2) * 3];
+        const float e0x = p1[0] - p0[0], e0y = p1[1] - p0[1], e0z = p1[2] - p0[2];
+        const float e1x = p2[0] - p1[0], e1y = p2[1] - p1[1], e1z = p2[2] - p1[2];
+        const float cx = std::fabs(e0y * e1z - e0z * e1y);
+        const float cy = std::fabs(e0z * e1x - e0x * e1z);
+        const float cz = std::fabs(e0x * e1y - e0y * e1x);
+        const float eps = std::numeric_limits<float>::epsilon();
+        if (cx > eps || cy > eps || cz > eps) {
+            if (!(cx > cy && cx > cz)) {
+                ax0 = 0;
+                if (cz > cx && cz > cy) ax1 = 1;
+            }
+            break;
+        }
+    }
+    std::vector<Corner> rem = face;
+    size_t guess = 0, budget = nf, lastSize = nf;
+    while (rem.size() > 3 && budget > 0) {
+        const size_t n = rem.size();
+        if (guess >= n) guess -= n;
+        if (lastSize != n) {
+            lastSize = n;
+            budget = n;
+        } else {
+            --budget;
+        }
+        Corner ear[3];
+        float x[3], y[3];
+        for (int k = 0; k < 3; ++k) {
+            ear[k] = rem[(guess + k) % n];
+            x[k] = coord(ear[k].v, ax0);
+            y[k] = coord(ear[k].v, ax1);
+        }
+        const float turn = (x[1] - x[0]) * (y[2] - y[1]) - (y[1] - y[0]) * (x[2] - x[1]);
+        const float fan = (x[0] * y[1] - y[0] * x[1]) * 0.5f;
+        if (turn * fan < 0.0f) {
+            ++guess;
+            continue;
+        }
+        bool blocked = false;
+        for (size_t o = 3; o < n && !blocked; ++o) {
+            const int vi = rem[(guess + o) % n].v;
+            if (vi < 0 || size_t(vi) * 3 + size_t(ax0) >= v.size() || size_t(vi) * 3 + size_t(ax1) >= v.size())
+                continue;
+            blocked = inside_polygon(3, x, y, v[size_t(vi) * 3 + size_t(ax0)], v[size_t(vi) * 3 + size_t(ax1)]);
+        }
+        if (blocked) {
+            ++guess;
+            continue;
+        }
+        tris.insert(tris.end(), ear, ear + 3);
+        rem.erase(rem.begin() + static_cast<std::ptrdiff_t>((guess + 1) % n));
+    }
+    if (rem.size() == 3) tris.insert(tris.end(), rem.begin(), rem.end());
+}
+
+}  // namespace
+
 bool load_obj(const std::string& path, MeshData& out, std::string& err) {
     std::ifstream f(path);
     if (!f) {
@@ -123,7 +260,8 @@ bool load_obj(const std::string& path, MeshData& out, std::string& err) {
         return false;
     }
     std::vector<float> v;  // all 'v' records
-    std::vector<std::vector<int>> faces;
+    int nvt = 0, nvn = 0;  // 'vt' and 'vn' records (for relative indices)
+    std::vector<std::vector<Corner>> faces;
     std::string line, acc;
     while (std::getline(f, line)) {
         if (!line.empty() && line.back() == '\r') line.pop_back();
@@ -143,22 +281,24 @@ bool load_obj(const std::string& path, MeshData& out, std::string& err) {
             v.push_back(x);
             v.push_back(y);
             v.push_back(z);
+        } else if (p[0] == 'v' && p[1] == 't' && (p[2] == ' ' || p[2] == '\t')) {
+            ++nvt;
+        } else if (p[0] == 'v' && p[1] == 'n' && (p[2] == ' ' || p[2] == '\t')) {
+            ++nvn;
         } else if (p[0] == 'f' && (p[1] == ' ' || p[1] == '\t')) {
             p += 2;
-            std::vector<int> fv;
+            std::vector<Corner> fv;
             const int nv = static_cast<int>(v.size() / 3);
             while (true) {
                 p += std::strspn(p, " \t");
                 if (*p == '\0') break;
                 const char* end = p + std::strcspn(p, " \t\r");
-                int idx = std::atoi(p);
-                if (idx == 0) {
+                Corner c;
+                if (!parse_corner(p, end, nv, nvt, nvn, c)) {
                     err = "Failed to parse `f' line (invalid vertex index) in " + path;
                     return false;
                 }
-                // 1-based, negative = relative to the current vertex count
-                int fixed = idx > 0 ? idx - 1 : nv + idx;
-                fv.push_back(fixed);
+                fv.push_back(c);
                 p = end;
             }
             if (fv.size() >= 3) faces.push_back(std::move(fv));
@@ -167,17 +307,28 @@ bool load_obj(const std::string& path, MeshData& out, std::string& err) {
     out.positions.clear();
     out.indices.clear();
     const int nv = static_cast<int>(v.size() / 3);
-    std::unordered_map<int, uint32_t> remap;
-    auto add = [&](int vi) -> bool {
-        if (vi < 0 || vi >= nv) return false;
-        auto it = remap.find(vi);
+    // vertices deduplicated by (position, normal, texcoord) in first-use order
+    struct Key {
+        int v, vn, vt;
+        bool operator==(const Key& o) const { return v == o.v && vn == o.vn && vt == o.vt; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const {
+            return std::hash<int>{}(k.v) ^ (std::hash<int>{}(k.vn) << 1) ^ (std::hash<int>{}(k.vt) << 2);
+        }
+    };
+    std::unordered_map<Key, uint32_t, KeyHash> remap;
+    auto add = [&](const Corner& c) -> bool {
+        if (c.v < 0 || c.v >= nv) return false;
+        const Key key{c.v, c.vn, c.vt};
+        auto it = remap.find(key);
         uint32_t id;
         if (it == remap.end()) {
             id = static_cast<uint32_t>(out.positions.size() / 3);
-            out.positions.push_back(v[3 * size_t(vi)]);
-            out.positions.push_back(v[3 * size_t(vi) + 1]);
-            out.positions.push_back(v[3 * size_t(vi) + 2]);
-            remap.emplace(vi, id);
+            out.positions.push_back(v[3 * size_t(c.v)]);
+            out.positions.push_back(v[3 * size_t(c.v) + 1]);
+            out.positions.push_back(v[3 * size_t(c.v) + 2]);
+            remap.emplace(key, id);
         } else {
             id = it->second;
         }
@@ -185,18 +336,19 @@ bool load_obj(const std::string& path, MeshData& out, std::string& err) {
         return true;
     };
     for (const auto& fv : faces) {
-        std::vector<int> tri;
+        std::vector<Corner> tri;
         if (fv.size() == 3) {
             tri = fv;
         } else if (fv.size() == 4) {
-            // split on the shorter diagonal (tinyOBJLoader.hpp:1509-1600)
+            // split on the shorter diagonal (tinyOBJLoader.hpp:1509-1600);
+            // a quad with an index out of range is skipped
             bool ok = true;
-            for (int k = 0; k < 4; ++k) ok = ok && fv[k] >= 0 && fv[k] < nv;
+            for (int k = 0; k < 4; ++k) ok = ok && fv[k].v >= 0 && fv[k].v < nv;
             if (!ok) continue;
-            const float* a = &v[3 * size_t(fv[0])];
-            const float* b = &v[3 * size_t(fv[1])];
-            const float* c = &v[3 * size_t(fv[2])];
-            const float* d = &v[3 * size_t(fv[3])];
+            const float* a = &v[3 * size_t(fv[0].v)];
+            const float* b = &v[3 * size_t(fv[1].v)];
+            const float* c = &v[3 * size_t(fv[2].v)];
+            const float* d = &v[3 * size_t(fv[3].v)];
             float e02x = c[0] - a[0], e02y = c[1] - a[1], e02z = c[2] - a[2];
             float e13x = d[0] - b[0], e13y = d[1] - b[1], e13z = d[2] - b[2];
             float s02 = e02x * e02x + e02y * e02y + e02z * e02z;
@@ -204,15 +356,10 @@ bool load_obj(const std::string& path, MeshData& out, std::string& err) {
             if (s02 < s13) tri = {fv[0], fv[1], fv[2], fv[0], fv[2], fv[3]};
             else tri = {fv[0], fv[1], fv[3], fv[1], fv[2], fv[3]};
         } else {
-            // polygons of 5+ vertices: fan from the first vertex
-            for (size_t k = 1; k + 1 < fv.size(); ++k) {
-                tri.push_back(fv[0]);
-                tri.push_back(fv[k]);
-                tri.push_back(fv[k + 1]);
-            }
+            clip_ears(fv, v, tri);
         }
-        for (int vi : tri)
-            if (!add(vi)) {
+        for (const Corner& c : tri)
+            if (!add(c)) {
                 err = "Face with invalid vertex index found in " + path;
                 return false;
             }

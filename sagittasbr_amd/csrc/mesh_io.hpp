@@ -3,10 +3,10 @@
 //
 // Behaviour follows the reference's modelLoader (src/modelLoader.cpp):
 //   OBJ  (loadObjInternal, :59-178, through tinyobjloader): faces are
-//        triangulated (quads split on the shorter diagonal as tinyobj does,
-//        larger polygons fanned), non-triangles after triangulation are
-//        skipped, vertices are deduplicated by (position, normal, texcoord)
-//        index triple in first-use order. Numbers are parsed with tinyobj's
+//        triangulated as tinyobj triangulates them (quads split on the
+//        shorter diagonal, larger polygons by its built-in ear clipping),
+//        vertices are deduplicated by (position, normal, texcoord) index
+//        triple in first-use order. Numbers are parsed with tinyobj's
 //        own accumulation (tinyOBJLoader.hpp:897-1028), which is not
 //        correctly rounded, then narrowed to float -- so positions are
 //        bit-identical to the reference's.

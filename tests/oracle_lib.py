@@ -288,6 +288,45 @@ REF_GPU_SO = {
     "f32": os.path.join(ROOT, "oracle", "_ref", "libref_gpu_f32.so"),
     "f64": os.path.join(ROOT, "oracle", "_ref", "libref_gpu_f64.so"),
 }
+REF_LOADER_SO = os.path.join(ROOT, "oracle", "_ref", "libref_loader.so")
+_ref_loader = None
+
+
+def ref_loader():
+    """The reference's own model loader (src/modelLoader.cpp with its
+    vendored tinyobjloader / tinygltf) built by oracle/Makefile; None when not
+    built."""
+    global _ref_loader
+    if _ref_loader is None and os.path.exists(REF_LOADER_SO):
+        L = C.CDLL(REF_LOADER_SO)
+        L.refload_open.restype = _P
+        L.refload_open.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        L.refload_info.argtypes = [_P, _lp]
+        L.refload_copy.argtypes = [_P, _fp, _up]
+        L.refload_close.argtypes = [_P]
+        _ref_loader = L
+    return _ref_loader
+
+
+def ref_load_model(path):
+    """(float32 positions (V, 3), uint32 faces (F, 3)) from the reference
+    loader; raises with its message when it fails."""
+    L = ref_loader()
+    err = C.create_string_buffer(4096)
+    h = L.refload_open(os.fsencode(path), err, 4096)
+    if not h:
+        raise ValueError(err.value.decode(errors="replace"))
+    try:
+        n = np.zeros(2, np.int64)
+        L.refload_info(h, n)
+        v = np.zeros(int(n[0]) * 3, np.float32)
+        f = np.zeros(int(n[1]) * 3, np.uint32)
+        L.refload_copy(h, v, f)
+    finally:
+        L.refload_close(h)
+    return v.reshape(-1, 3), f.reshape(-1, 3)
+
+
 _ref_gpu = {}
 
 
