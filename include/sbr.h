@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SBR_ABI_VERSION 3
+#define SBR_ABI_VERSION 4
 
 typedef struct sbr_ctx sbr_ctx;
 typedef struct sbr_plan sbr_plan;
@@ -93,8 +93,12 @@ typedef struct {
     int traversal;   /* loop shape of the BVH traversal: 1 = if-if (a wavefront runs node
                         steps and leaf tests side by side), 2 = while-while (leaves are
                         tested once every lane has reached one; default) */
-    int top_levels;  /* SBR_MIXED, one ray per lane: levels of the four-wide tree each
-                        workgroup stages in LDS (1..4; 0 = none, default) */
+    int top_levels;  /* SBR_MIXED, one ray per lane: levels of the wide tree each
+                        workgroup stages in LDS (1..4, as many records as fit 10,880
+                        bytes; 0 = none, default) */
+    int node_width;  /* SBR_MIXED, one ray per lane: children per record of the FP32 box
+                        tree the mixed query walks: 2 (the binary tree), 4 (default) or 8;
+                        a width is built (allocating) the first time a plan uses it */
 } sbr_tuning;
 
 /* Mesh / BVH statistics. */
@@ -217,7 +221,8 @@ int sbr_plan_destroy(sbr_plan* plan);
  * x tile side {16, 32} x hipGraph replay {0, 1} (per-angle); then the other
  * traversal loop shape at the best of those; then, for SBR_MIXED, the top 3
  * or 4 tree levels staged in LDS at that shape and with an 8-entry LDS
- * stack. Keeps the fastest and applies it to the plan. sample_jobs > 0
+ * stack, then node widths 2 and 8 at the best so far. Keeps the fastest and
+ * applies it to the plan. sample_jobs > 0
  * tunes on an evenly strided subset of that many jobs (for large sweeps);
  * repeats >= 1 timed runs per candidate (the minimum wall time counts).
  * Results do not depend on the tuning: partial sums are kept per 16 x 16 ray

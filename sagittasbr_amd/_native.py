@@ -91,9 +91,12 @@ class Job(C.Structure):
     ]
 
 
+ABI_VERSION = 4  # SBR_ABI_VERSION of the structures below
+
+
 class Tuning(C.Structure):
     _fields_ = [("wg_per_cu", C.c_int), ("lds_stack", C.c_int), ("tile_rays", C.c_int), ("use_graph", C.c_int),
-                ("ray_pool", C.c_int), ("traversal", C.c_int), ("top_levels", C.c_int)]
+                ("ray_pool", C.c_int), ("traversal", C.c_int), ("top_levels", C.c_int), ("node_width", C.c_int)]
 
 
 class MeshInfo(C.Structure):
@@ -185,6 +188,8 @@ def lib():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+    if L.sbr_abi_version() != ABI_VERSION:
+        raise SbrError(f"{path} has ABI {L.sbr_abi_version()}, this binding expects {ABI_VERSION}: rebuild it")
     _lib = L
     return L
 
@@ -289,8 +294,9 @@ class Plan:
         self.h = _P()
         _check(lib().sbr_plan_create(ctx.h, C.cast(jobs, _P), self.n, precision, method, C.byref(self.h)), ctx.h)
 
-    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0, ray_pool=0, traversal=0, top_levels=0):
-        t = Tuning(wg_per_cu, lds_stack, tile_rays, use_graph, ray_pool, traversal, top_levels)
+    def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0, ray_pool=0, traversal=0, top_levels=0,
+             node_width=0):
+        t = Tuning(wg_per_cu, lds_stack, tile_rays, use_graph, ray_pool, traversal, top_levels, node_width)
         _check(lib().sbr_plan_set_tuning(self.h, C.byref(t)), self.ctx.h)
 
     def autotune(self, sample_jobs=0, repeats=1):
@@ -301,7 +307,7 @@ class Plan:
                self.ctx.h)
         return {"wg_per_cu": t.wg_per_cu, "lds_stack": t.lds_stack, "tile_rays": t.tile_rays,
                 "use_graph": t.use_graph, "ray_pool": t.ray_pool, "traversal": t.traversal,
-                "top_levels": t.top_levels, "best_ms": ms.value,
+                "top_levels": t.top_levels, "node_width": t.node_width, "best_ms": ms.value,
                 "candidates": n.value}
 
     def run(self):

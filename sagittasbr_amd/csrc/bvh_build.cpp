@@ -349,9 +349,10 @@ template <typename R> bool to_gpu_layout(const Bvh<R>& b, GpuBvh<R>& out, std::s
     return true;
 }
 
-bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
-    constexpr int W = kWide;
-    constexpr int F = kWideRecBytes / 4;  // floats per record
+namespace {
+template <int W> bool to_gpu_layout_w(const Bvh<float>& b, GpuBvhW& out, std::string& err) {
+    constexpr int F = kRecBytes<W> / 4;  // floats per record
+    out.width = W;
     const int nn = static_cast<int>(b.nodes.size());
     if (nn == 0) {
         err = "empty mesh";
@@ -364,7 +365,7 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
     };
     auto leaf_ref = [&](int i) { return encode_leaf(b.nodes[i].firstTri, b.nodes[i].triCount); };
     if (b.nodes[b.root].triCount != 0) {
-        out.nodes.assign(kWideRecBytes, 0);
+        out.nodes.assign(kRecBytes<W>, 0);
         out.rootRef = leaf_ref(b.root);
         out.nInternal = 0;
         out.depth = 0;
@@ -455,7 +456,7 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
             newIdx[r] = static_cast<int>(order.size());
             order.push_back(r);
         }
-    out.nodes.resize(size_t(nr) * kWideRecBytes);
+    out.nodes.resize(size_t(nr) * kRecBytes<W>);
     for (int k = 0; k < nr; ++k) {
         std::array<float, F> r = recs[order[k]];
         for (int q = 0; q < W; ++q) {
@@ -464,12 +465,22 @@ bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err) {
             if (ref >= 0 && ref != kEmptySlot) ref = newIdx[ref];
             std::memcpy(&r[6 * W + q], &ref, sizeof(int));
         }
-        std::memcpy(out.nodes.data() + size_t(k) * kWideRecBytes, r.data(), kWideRecBytes);
+        std::memcpy(out.nodes.data() + size_t(k) * kRecBytes<W>, r.data(), kRecBytes<W>);
     }
     out.rootRef = newIdx[0];
     out.nInternal = nr;
     out.depth = maxDepth;
     return true;
+}
+}  // namespace
+
+bool to_gpu_layout_wide(const Bvh<float>& b, int width, GpuBvhW& out, std::string& err) {
+    switch (width) {
+        case 2: return to_gpu_layout_w<2>(b, out, err);
+        case 4: return to_gpu_layout_w<4>(b, out, err);
+        case 8: return to_gpu_layout_w<8>(b, out, err);
+        default: err = "node width must be 2, 4 or 8"; return false;
+    }
 }
 
 template <typename R> std::vector<float> cull_boxes(const Bvh<R>& b, int maxBoxes) {

@@ -62,11 +62,13 @@ template <typename R> bool to_gpu_layout(const Bvh<R>& b, GpuBvh<R>& out, std::s
 // Wide FP32 layout for the mixed mode's certified queries, whose outcome
 // does not depend on the tree or its visiting order (only on which triangles
 // are tested). Made by collapsing the binary tree: each record takes up to
-// kWide descendants, opening the child with the largest surface area first;
-// leaves are the binary tree's leaves (same references). Record of
-// kWideRecBytes: lo.x[W] lo.y[W] lo.z[W] hi.x[W] hi.y[W] hi.z[W] ref[W], pad;
-// an unused slot holds ref kEmptySlot and a box at +infinity on every axis.
-struct GpuBvh4 {
+// W descendants (W = 2, 4 or 8), opening the child with the largest surface
+// area first; leaves are the binary tree's leaves (same references). Record
+// of kRecBytes<W>: lo.x[W] lo.y[W] lo.z[W] hi.x[W] hi.y[W] hi.z[W] ref[W],
+// pad; an unused slot holds ref kEmptySlot and a box at +infinity on every
+// axis.
+struct GpuBvhW {
+    int width = kWide;
     std::vector<uint8_t> nodes;
     int rootRef = 0;
     int nInternal = 0;
@@ -75,7 +77,8 @@ struct GpuBvh4 {
     // order: topRecs[k] records hold levels 0 .. k-1
     int topRecs[kTopLevels + 1] = {};
 };
-bool to_gpu_layout4(const Bvh<float>& b, GpuBvh4& out, std::string& err);
+// width: 2, 4 or 8
+bool to_gpu_layout_wide(const Bvh<float>& b, int width, GpuBvhW& out, std::string& err);
 
 // Up to maxBoxes boxes that together hold every triangle: the binary tree's
 // frontier reached by opening the largest-area internal node first. Written

@@ -26,14 +26,14 @@
 namespace sbr {
 template <typename R>
 hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool, bool ww,
-                        bool top, int grid, hipStream_t s);
+                        bool top, int width, int grid, hipStream_t s);
 hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s);
 template <typename R>
 hipError_t launch_finalize(const Job<R>* jobs, int nJobs, int mode, const void* partials, double* jobOut,
                            hipStream_t s);
 size_t partial_bytes(bool fp64, int mode);
 hipError_t launch_adjoint(const AdjArgs& a, int blocks, hipStream_t s);
-template <typename R> int trace_occupancy(int mode, int sd, bool pool, bool top);
+template <typename R> int trace_occupancy(int mode, int sd, bool pool, bool top, int width);
 hipError_t scratch_init(int device);
 void* scratch_ptr(int device);
 }  // namespace sbr
@@ -93,8 +93,9 @@ template <typename R> struct PrecMesh {
     DevBuf nodes, tris;
     DevBuf tris64;  // FP32 tree: FP64 vertex records in tree order (80 B each), unless vertsExact
     bool vertsExact = false;  // FP32 tree: every FP64 vertex is its FP32 vertex widened
-    GpuBvh4 gpu4;   // FP32 tree: four-wide collapse (mixed mode)
-    DevBuf nodes4;
+    GpuBvhW gpuW[3];  // FP32 tree: 2-, 4- and 8-wide collapses (mixed mode; width_slot order)
+    DevBuf nodesW[3];
+    bool builtW[3] = {false, false, false};
     DevBuf cull;    // boxes of the per-tile emptiness test
     int nCull = 0;
     double buildMs = 0;
@@ -142,7 +143,7 @@ struct sbr_plan {
     DevBuf dChunkJob;  // job of the first tile of each kChunkTiles chunk
     int64_t nChunks = 0;
     int maxDepth = 0;
-    sbr_tuning tune{0, 0, 0, 0, 0, 0, 0};
+    sbr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0};
     int tileSide = 32;
     DevBuf dJobs, dPartials, dJobOut, dCounter, dCounters;
     // mixed mode
@@ -170,22 +171,40 @@ float widen_ulps(float v, int n) {
     for (int k = 0; k < (n < 0 ? -n : n); ++k) v = std::nextafter(v, n < 0 ? -INFINITY : INFINITY);
     return v;
 }
-void widen_boxes4(GpuBvh4& g) {
-    const size_t nRec = g.nodes.size() / kWideRecBytes;
+void widen_boxes(GpuBvhW& g) {
+    const int W = g.width;
+    const size_t rb = size_t(W * 28 + 63) / 64 * 64;
+    const size_t nRec = g.nodes.size() / rb;
+    std::vector<float> f(rb / 4);
     for (size_t i = 0; i < nRec; ++i) {
-        float f[kWideRecBytes / 4];
-        std::memcpy(f, g.nodes.data() + i * kWideRecBytes, kWideRecBytes);
-        for (int q = 0; q < kWide; ++q) {
+        std::memcpy(f.data(), g.nodes.data() + i * rb, rb);
+        for (int q = 0; q < W; ++q) {
             int ref;
-            std::memcpy(&ref, &f[6 * kWide + q], sizeof(int));
+            std::memcpy(&ref, &f[6 * W + q], sizeof(int));
             if (ref == kEmptySlot) continue;
             for (int k = 0; k < 3; ++k) {
-                f[k * kWide + q] = widen_ulps(f[k * kWide + q], -kWidenUlps);
-                f[(3 + k) * kWide + q] = widen_ulps(f[(3 + k) * kWide + q], kWidenUlps);
+                f[k * W + q] = widen_ulps(f[k * W + q], -kWidenUlps);
+                f[(3 + k) * W + q] = widen_ulps(f[(3 + k) * W + q], kWidenUlps);
             }
         }
-        std::memcpy(g.nodes.data() + i * kWideRecBytes, f, kWideRecBytes);
+        std::memcpy(g.nodes.data() + i * rb, f.data(), rb);
     }
+}
+
+// The mixed mode's wide collapse of the mesh's FP32 tree at this width,
+// built (on the host, then uploaded) on first use.
+int ensure_width(sbr_ctx* ctx, PrecMesh<float>& pm, int width) {
+    const int slot = width_slot(width);
+    if (pm.builtW[slot]) return 0;
+    std::string err;
+    if (!to_gpu_layout_wide(pm.bvh, width, pm.gpuW[slot], err)) return fail(&ctx->err, err);
+    if (!pm.vertsExact) widen_boxes(pm.gpuW[slot]);
+    HIP_TRY(&ctx->err, pm.nodesW[slot].ensure(pm.gpuW[slot].nodes.size()));
+    HIP_TRY(&ctx->err, hipMemcpy(pm.nodesW[slot].p, pm.gpuW[slot].nodes.data(), pm.gpuW[slot].nodes.size(),
+                                 hipMemcpyHostToDevice));
+    pm.builtW[slot] = true;
+    ctx->tableDirty32 = true;
+    return 0;
 }
 
 template <typename R> int upload_table(sbr_ctx* ctx);
@@ -237,11 +256,8 @@ template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
             std::memcpy(&t64[10 * h + 9], &fid, sizeof(double));
         }
         pm.vertsExact = exact;
-        if (!to_gpu_layout4(pm.bvh, pm.gpu4, err)) return fail(&ctx->err, err);
-        if (!exact) widen_boxes4(pm.gpu4);
-        HIP_TRY(&ctx->err, pm.nodes4.ensure(pm.gpu4.nodes.size()));
-        HIP_TRY(&ctx->err,
-                hipMemcpy(pm.nodes4.p, pm.gpu4.nodes.data(), pm.gpu4.nodes.size(), hipMemcpyHostToDevice));
+        // the default four-wide collapse now; other widths when a plan asks
+        if (ensure_width(ctx, pm, kWide)) return 1;
         if (!exact) {
             HIP_TRY(&ctx->err, pm.tris64.ensure(t64.size() * sizeof(double)));
             HIP_TRY(&ctx->err,
@@ -290,9 +306,11 @@ template <typename R> int upload_table(sbr_ctx* ctx) {
             mm.cull = static_cast<const float*>(pm.cull.p);
             mm.nCull = pm.nCull;
             if constexpr (std::is_same_v<R, float>) {
-                mm.nodes4 = pm.nodes4.p;
-                for (int k = 0; k <= kTopLevels; ++k) mm.topRecs4[k] = pm.gpu4.topRecs[k];
-                mm.rootRef4 = pm.gpu4.rootRef;
+                for (int w = 0; w < 3; ++w) {
+                    mm.nodesW[w] = pm.builtW[w] ? pm.nodesW[w].p : nullptr;
+                    mm.rootRefW[w] = pm.gpuW[w].rootRef;
+                    for (int k = 0; k <= kTopLevels; ++k) mm.topRecsW[w][k] = pm.gpuW[w].topRecs[k];
+                }
                 mm.vertsExact = pm.vertsExact ? 1 : 0;
                 for (int k = 0; k < 3; ++k) {
                     mm.rootLo4[k] = pm.vertsExact ? pm.gpu.rootLo[k] : widen_ulps(pm.gpu.rootLo[k], -kWidenUlps);
@@ -516,9 +534,16 @@ template <typename R> int plan_host(sbr_plan* pl, bool record, bool noAlloc, Hos
         } else {
             // reference FP64 job: the ray state and the FP64 answers
             if constexpr (std::is_same_v<R, float>) {
-                if (pm.gpu4.depth > kMaxDepth4)
-                    return fail(&ctx->err, "mixed mode: four-wide tree depth " + std::to_string(pm.gpu4.depth) +
-                                               " exceeds " + std::to_string(kMaxDepth4));
+                const int width = pl->tune.node_width ? pl->tune.node_width : kWide;
+                if (noAlloc && !pm.builtW[width_slot(width)])
+                    return fail(&ctx->err, "mesh " + std::to_string(j.mesh) + " has no " + std::to_string(width) +
+                                               "-wide tree yet (building it allocates)");
+                if (ensure_width(ctx, pm, width)) return 1;
+                const int depth = pm.gpuW[width_slot(width)].depth;
+                if (depth > max_depth_wide(width))
+                    return fail(&ctx->err, "mixed mode: " + std::to_string(width) + "-wide tree depth " +
+                                               std::to_string(depth) + " exceeds " +
+                                               std::to_string(max_depth_wide(width)));
             }
             if (noAlloc && !m.f64.built)
                 return fail(&ctx->err, "mesh " + std::to_string(j.mesh) +
@@ -728,9 +753,11 @@ template <typename R> int plan_enqueue(sbr_plan* pl, const DevPlan& d, bool reco
     a.recReIm = rReIm;
     a.counters = d.counters;
     const bool mixed = pl->precision == SBR_MIXED;
-    // top levels of the four-wide tree staged in LDS: mixed timed runs, one ray per lane
+    // top levels of the wide tree staged in LDS: mixed timed runs, one ray per lane
     const bool top = mixed && !record && !count && !pool && pl->tune.top_levels > 0;
     a.topLevels = top ? pl->tune.top_levels : 0;
+    // width of the mixed mode's box tree (the ray pool walks the default one)
+    const int width = mixed && !pool && pl->tune.node_width ? pl->tune.node_width : kWide;
     if (mixed) {
         a.jobs64 = static_cast<const Job<double>*>(d.jobs64);
         a.meshes64 = static_cast<const Mesh<double>*>(ctx->meshTable64.p);
@@ -749,20 +776,20 @@ template <typename R> int plan_enqueue(sbr_plan* pl, const DevPlan& d, bool reco
             if (pl->totalTiles > INT32_MAX) return fail(&ctx->err, "record run too large for one launch");
             a.tileOffset = 0;
             HIP_TRY(&ctx->err,
-                    launch_trace<R>(a, mode, sd, true, count, false, pool, ww, false, int(pl->totalTiles), s));
+                    launch_trace<R>(a, mode, sd, true, count, false, pool, ww, false, width, int(pl->totalTiles), s));
         } else if (pl->method == SBR_BATCHED) {
-            int occ = trace_occupancy<R>(mode, sd, pool, top);
+            int occ = trace_occupancy<R>(mode, sd, pool, top, width);
             int per = pl->tune.wg_per_cu > 0 ? pl->tune.wg_per_cu : std::max(occ, 1);
             int64_t grid = int64_t(ctx->cuCount) * per;
             grid = std::min<int64_t>(grid, pl->totalTiles);
             a.tileOffset = 0;
-            HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, false, count, true, pool, ww, top, int(grid), s));
+            HIP_TRY(&ctx->err, launch_trace<R>(a, mode, sd, false, count, true, pool, ww, top, width, int(grid), s));
         } else {
             for (int i = 0; i < pl->nJobs; ++i) {
                 if (pl->nTiles[i] == 0) continue;
                 a.tileOffset = pl->tileBase[i];
                 HIP_TRY(&ctx->err,
-                        launch_trace<R>(a, mode, sd, record, count, false, pool, ww, top, pl->nTiles[i], s));
+                        launch_trace<R>(a, mode, sd, record, count, false, pool, ww, top, width, pl->nTiles[i], s));
             }
         }
     }
@@ -1277,14 +1304,28 @@ int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
     if (t->traversal < 0 || t->traversal > 2) return fail(&pl->ctx->err, "traversal must be 0, 1 or 2");
     if (t->top_levels < 0 || t->top_levels > kTopLevels)
         return fail(&pl->ctx->err, "top_levels must be 0 .. " + std::to_string(kTopLevels));
+    if (t->node_width != 0 && t->node_width != 2 && t->node_width != 4 && t->node_width != 8)
+        return fail(&pl->ctx->err, "node_width must be 0, 2, 4 or 8");
     if (t->tile_rays != 0 && t->tile_rays != 16 && t->tile_rays != 32 && t->tile_rays != 64)
         return fail(&pl->ctx->err, "tile_rays must be 16, 32 or 64");
     drop_graph(pl);
+    const sbr_tuning old = pl->tune;
+    const int oldSide = pl->tileSide;
     pl->tune = *t;
     int ts = t->tile_rays ? t->tile_rays : default_tile(pl->method);
-    if (ts != pl->tileSide) {
+    // a new tile side changes the job tiling; a new node width (mixed mode)
+    // may build that width's tree and must fit the traversal stack
+    const bool widthChanged =
+        pl->precision == SBR_MIXED && (old.node_width ? old.node_width : kWide) != (t->node_width ? t->node_width : kWide);
+    if (ts != pl->tileSide || widthChanged) {
         pl->tileSide = ts;
-        if (dispatch_prepare(pl, false)) return 1;
+        if (dispatch_prepare(pl, false)) {
+            const std::string msg = pl->ctx->err;
+            pl->tune = old;
+            pl->tileSide = oldSide;
+            (void)dispatch_prepare(pl, false);
+            return fail(&pl->ctx->err, msg);
+        }
     }
     return 0;
 }
@@ -1371,10 +1412,10 @@ int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* be
         for (int sd : {8, 16, 32}) {
             if (pl->method == SBR_BATCHED) {
                 for (int ts : {16, 32, 64})
-                    for (int wg : {0, 2}) cands.push_back(sbr_tuning{wg, sd, ts, 0, pool, 0, 0});
+                    for (int wg : {0, 2}) cands.push_back(sbr_tuning{wg, sd, ts, 0, pool, 0, 0, 0});
             } else {
                 for (int ts : {16, 32})
-                    for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool, 0, 0});
+                    for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool, 0, 0, 0});
             }
         }
     double bestT = 1e300;
@@ -1421,6 +1462,15 @@ int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* be
                 c.top_levels = k;
                 if (!rc) rc = trial(c);
             }
+    }
+    // stage 4 (mixed, one ray per lane): the binary and the eight-wide tree
+    if (!rc && pl->precision == SBR_MIXED && bestC.ray_pool != 1) {
+        const sbr_tuning base = bestC;
+        for (int w : {2, 8}) {
+            sbr_tuning c = base;
+            c.node_width = w;
+            if (!rc) rc = trial(c);
+        }
     }
     if (owned) sbr_plan_destroy(owned);
     if (rc) return rc;

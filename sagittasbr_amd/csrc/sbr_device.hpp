@@ -11,18 +11,24 @@ constexpr int kMaxStack = 64;  // deferred-left capacity; equals the reference's
                                // int stack[64] (src/cuda/rtKernels.cu:65-66)
 constexpr int kMaxDepth = 62;  // deepest tree for which the reference never
                                // drops a push (stackPtr + 2 <= 64)
-// Mixed mode's wide tree: kWide children per record, up to kWide - 1 pushes
-// per record visited.
+// Mixed mode's wide tree: W children per record (2, 4 or 8; kWide by
+// default), up to W - 1 pushes per record visited.
 constexpr int kWide = 4;
-constexpr int kWideRecBytes = (kWide * 28 + 63) / 64 * 64;  // 6 box floats + 1 ref per child, padded
-constexpr int kMaxStack4 = 96;
+template <int W> constexpr int kRecBytes = (W * 28 + 63) / 64 * 64;  // 6 box floats + 1 ref per child, padded
+constexpr int kWideRecBytes = kRecBytes<kWide>;
+constexpr int kMaxStack4 = 96;  // deferred children of one mixed query (any width)
 constexpr int kMaxDepth4 = kMaxStack4 / (kWide - 1);
+// deepest wide tree of width w whose deferred children fit kMaxStack4
+constexpr int max_depth_wide(int w) { return kMaxStack4 / (w - 1); }
+// index of width 2, 4, 8 in per-width arrays
+constexpr int width_slot(int w) { return w == 2 ? 0 : (w == 8 ? 2 : 1); }
 constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a wide record
 constexpr int kCullBoxes = 256;         // boxes of the tile and ray-patch emptiness tests
-// Top levels of the four-wide tree that a workgroup can stage in LDS: at
-// most 1 + 4 + 16 + 64 = 85 records of kWideRecBytes
+// Top levels of the wide tree that a workgroup can stage in LDS: at most
+// kTopBytes of records (1 + 4 + 16 + 64 = 85 four-wide records)
 constexpr int kTopLevels = 4;
 constexpr int kTopRecs = 85;
+constexpr int kTopBytes = kTopRecs * kWideRecBytes;
 constexpr int kChunkTiles = 64;         // tiles per entry of the tile -> job table
 // Fixed device scratch of the allocation-free sweep (sbr_sweep_noalloc)
 constexpr size_t kScratchBytes = size_t(64) << 20;
@@ -68,11 +74,13 @@ template <typename R> struct alignas(16) Mesh {
     // (model scale 1, or any scale that rounds no vertex), so the mixed
     // mode reads the FP32 record and widens it exactly
     int vertsExact;
-    // FP32 tree only: the four-wide collapse of the tree (mixed mode)
-    const void* nodes4;
-    // FP32 tree only: records of the four-wide tree's top k levels, which
-    // come first (topRecs4[k], k = 0 .. kTopLevels)
-    int topRecs4[kTopLevels + 1];
+    // FP32 tree only: the two-, four- and eight-wide collapses of the tree
+    // (mixed mode; width_slot order; a width not built yet is null), their
+    // root records, and the records of their top k levels, which come first
+    // (topRecsW[slot][k], k = 0 .. kTopLevels)
+    const void* nodesW[3];
+    int rootRefW[3];
+    int topRecsW[3][kTopLevels + 1];
     // up to kCullBoxes boxes (lo.xyz hi.xyz, FP32, rounded outwards) that
     // together hold every triangle, for the per-tile emptiness test
     const float* cull;
@@ -85,7 +93,6 @@ template <typename R> struct alignas(16) Mesh {
     // (mixed mode's tie margin scale)
     double tieScale;
     int rootRef;
-    int rootRef4;
     int nTris;
 };
 
@@ -119,7 +126,7 @@ template <typename R> struct TraceArgs {
     const Mesh<double>* meshes64;
     float errK;
     int8_t* recFlag;
-    // levels of the four-wide tree staged in LDS (mixed mode, 1 .. kTopLevels)
+    // levels of the wide tree staged in LDS (mixed mode, 1 .. kTopLevels)
     int topLevels;
 };
 

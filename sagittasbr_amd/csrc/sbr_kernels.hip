@@ -28,7 +28,8 @@
 // and square root are IEEE correctly rounded (the mixed mode's FP32 slab
 // reciprocals excepted: their error is part of its box bound).
 //
-// Mixed mode (MODE 1): FP32 box traversal of a four-wide tree against an
+// Mixed mode (MODE 1): FP32 box traversal of a wide tree (2, 4 or 8
+// children per record; four by default) against an
 // FP32 copy of the FP64 reference ray, every box grown by that copy's
 // position error bound (so pruning is conservative), and the reference FP64
 // Moller-Trumbore test of every triangle reached. A query whose answer could
@@ -202,16 +203,19 @@ __device__ __forceinline__ NodeRec<double> load_node(const double* nodes, int i)
 }
 
 // Wide record of the mixed mode's tree: lo.x[W] lo.y[W] lo.z[W] hi.x[W]
-// hi.y[W] hi.z[W] ref[W] as 16-byte loads.
-struct NodeW {
-    float f[7 * kWide];
-    __device__ __forceinline__ int ref(int q) const { return __float_as_int(f[6 * kWide + q]); }
+// hi.y[W] hi.z[W] ref[W] as 16-byte loads (W = 2: the last 8 bytes are
+// the record's padding).
+template <int W> struct NodeW {
+    static constexpr int kLoads = (7 * W + 3) / 4;
+    static_assert(16 * kLoads <= kRecBytes<W>, "loads stay in the record");
+    float f[4 * kLoads];
+    __device__ __forceinline__ int ref(int q) const { return __float_as_int(f[6 * W + q]); }
 };
-__device__ __forceinline__ NodeW load_nodew(const void* nodes, int i) {
-    gptr<vf4> p = as_global<vf4>(nodes) + (kWideRecBytes / 16) * i;
-    NodeW n;
+template <int W> __device__ __forceinline__ NodeW<W> load_nodew(const void* nodes, int i) {
+    gptr<vf4> p = as_global<vf4>(nodes) + (kRecBytes<W> / 16) * i;
+    NodeW<W> n;
 #pragma unroll
-    for (int k = 0; k < 7 * kWide / 4; ++k) {
+    for (int k = 0; k < NodeW<W>::kLoads; ++k) {
         const vf4 v = p[k];
         n.f[4 * k] = v.x;
         n.f[4 * k + 1] = v.y;
@@ -222,12 +226,12 @@ __device__ __forceinline__ NodeW load_nodew(const void* nodes, int i) {
 }
 
 // The same record from the workgroup's LDS copy of the tree's top levels
-// (records of kWideRecBytes, 16-byte aligned).
-__device__ __forceinline__ NodeW load_nodew_lds(const vf4* top, int i) {
-    const vf4* p = top + (kWideRecBytes / 16) * i;
-    NodeW n;
+// (records of kRecBytes<W>, 16-byte aligned).
+template <int W> __device__ __forceinline__ NodeW<W> load_nodew_lds(const vf4* top, int i) {
+    const vf4* p = top + (kRecBytes<W> / 16) * i;
+    NodeW<W> n;
 #pragma unroll
-    for (int k = 0; k < 7 * kWide / 4; ++k) {
+    for (int k = 0; k < NodeW<W>::kLoads; ++k) {
         const vf4 v = p[k];
         n.f[4 * k] = v.x;
         n.f[4 * k + 1] = v.y;
@@ -249,10 +253,12 @@ template <int W> __device__ __forceinline__ void sort_net(float* e, int* r) {
         r[a] = ra;
         r[b] = rb;
     };
-    if constexpr (W == 4) {
+    if constexpr (W == 2) {
+        cs(0, 1);
+    } else if constexpr (W == 4) {
         cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
     } else {
-        static_assert(W == 8, "wide records of 4 or 8 children");
+        static_assert(W == 8, "wide records of 2, 4 or 8 children");
         cs(0, 1); cs(2, 3); cs(4, 5); cs(6, 7);
         cs(0, 2); cs(1, 3); cs(4, 6); cs(5, 7);
         cs(1, 2); cs(5, 6);
@@ -450,7 +456,7 @@ constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
 // along the ray by shift (first query): FP32 box distances are then shift
 // shorter than the FP64 ones. Unless amb is set, a hit's FP64 vertices and
 // face id are returned in hv0, hv1, hv2 and hFace.
-template <int SD, bool CNT, bool WW, bool TOP>
+template <int SD, bool CNT, bool WW, bool TOP, int W>
 __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> inv, V3<double> o64,
                                                  V3<double> d64, double shift, double& tHit, int* sRef, float* sE,
                                                  Ov<float>::T* ov, const vf4* top, int nTop, int tid, Work& w,
@@ -488,13 +494,13 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     float lim = 3.0e38f;
     auto update_lim = [&]() { lim = float(closest - shift) * (1.0f + 4.0f * kEps32) + 2.0f * tErr; };
     LaneStack<float, SD> st{sRef, sE, ov, tid, 0};
-    int cur = m.rootRef4;
+    const void* nodes = m.nodesW[width_slot(W)];
+    int cur = m.rootRefW[width_slot(W)];
     // wide tree: the hit children are visited nearest first (the outcome
     // does not depend on the order)
     auto node_step = [&]() -> bool {
-        constexpr int W = kWide;
         // records below nTop are the top levels the workgroup staged in LDS
-        const NodeW n = (TOP && cur < nTop) ? load_nodew_lds(top, cur) : load_nodew(m.nodes4, cur);
+        const NodeW<W> n = (TOP && cur < nTop) ? load_nodew_lds<W>(top, cur) : load_nodew<W>(nodes, cur);
         if (CNT) w.visits += W;
         float e[W];
         int r[W];
@@ -841,7 +847,7 @@ __device__ __forceinline__ void ray_init_mixed(const Job<double>& D, int i, int 
 // (a near-tie or a hit on the edge of its box) is answered by the reference
 // FP64 query on the FP64 tree from the same FP64 ray state -- the reference
 // query itself -- and sets fell.
-template <int SD, bool CNT, bool REC, bool WW, bool TOP>
+template <int SD, bool CNT, bool REC, bool WW, bool TOP, int W>
 __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<float>& M, const Mesh<double>& M64,
                                                RayState<double>& st, int* sRef, float* sE, Ov<float>::T* ov,
                                                Ov<double>::T* ov64, const vf4* top, int nTop, int tid, int32_t* ids,
@@ -868,7 +874,7 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
     bool amb = false;
     V3<double> v0, v1, v2, e1, e2;
     int face;
-    int h = closest_hit_mixed<SD, CNT, WW, TOP>(M, of, inv, o, d, shift, t, sRef, sE, ov, top, nTop, tid, w, ex, ed,
+    int h = closest_hit_mixed<SD, CNT, WW, TOP, W>(M, of, inv, o, d, shift, t, sRef, sE, ov, top, nTop, tid, w, ex, ed,
                                                 K, amb, v0, v1, v2, face);
     if (CNT) queries++;
     // FP64 edges of the chosen triangle (rtKernels.cu:13-14)
@@ -906,7 +912,7 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
 
 // Whole mixed-mode path in one call; fell is set when a query was answered
 // on the FP64 tree.
-template <int SD, bool CNT, bool REC, bool WW, bool TOP>
+template <int SD, bool CNT, bool REC, bool WW, bool TOP, int W>
 __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J, const Job<double>& D,
                                                             const Mesh<float>& M, const Mesh<double>& M64, int i,
                                                             int j, int* sRef, float* sE, Ov<float>::T* ov,
@@ -917,7 +923,7 @@ __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J,
     RayState<double> st;
     ray_init_mixed(D, i, j, st);
     if (J.maxB > 0)
-        while (!ray_step_mixed<SD, CNT, REC, WW, TOP>(J, M, M64, st, sRef, sE, ov, ov64, top, nTop, tid, ids, w,
+        while (!ray_step_mixed<SD, CNT, REC, WW, TOP, W>(J, M, M64, st, sRef, sE, ov, ov64, top, nTop, tid, ids, w,
                                                       queries, fullQ, K, fell)) {
         }
     st.P.Ld = st.P.L;
@@ -1005,10 +1011,12 @@ constexpr int kPoolRays = 1024;  // rays per pool batch (4 passes of 16x16)
 template <typename R, int MODE> constexpr int kWavesPerEu =
     MODE == 1 ? SBR_WPE_MIXED : (std::is_same_v<R, double> ? SBR_WPE_FP64 : SBR_WPE_FP32);
 
-template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW, bool TOP = false>
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW, bool TOP = false,
+          int W = kWide>
 __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<R, MODE>))) void trace_kernel(
     TraceArgs<R> a) {
     static_assert(!TOP || (MODE == 1 && !REC && !CNT && !POOL), "top-level staging: mixed mode, timed runs");
+    static_assert(W == kWide || (MODE == 1 && !POOL), "node widths other than the default: mixed mode, no pool");
     using ST = typename SlotOf<R, MODE>::T;
     using RS = typename std::conditional<MODE == 1, double, R>::type;  // ray state precision
     __shared__ int sRef[SD * kWgSize];
@@ -1021,8 +1029,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     __shared__ unsigned sPool;
     // projected cull boxes of the current tile's job (one per thread)
     __shared__ CullRect sCull[(REC || CNT) ? 1 : kCullBoxes];
-    // the top levels of the current mesh's four-wide tree (TOP)
-    __shared__ vf4 sTop[TOP ? kTopRecs * (kWideRecBytes / 16) : 1];
+    // the top levels of the current mesh's W-wide tree (TOP)
+    __shared__ vf4 sTop[TOP ? kTopBytes / 16 : 1];
     int stagedMesh = -1, nTop = 0;
     static_assert(kCullBoxes == kWgSize, "one cull box per thread");
 
@@ -1083,9 +1091,11 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
             // by the barrier below)
             if (J.mesh != stagedMesh) {
                 stagedMesh = J.mesh;
-                nTop = min(M.topRecs4[a.topLevels], kTopRecs);
-                const gptr<vf4> src = as_global<vf4>(M.nodes4);
-                for (int k = tid; k < nTop * (kWideRecBytes / 16); k += kWgSize) sTop[k] = src[k];
+                // (a prefix of the breadth-first top records when the levels
+                // do not fit)
+                nTop = min(M.topRecsW[width_slot(W)][a.topLevels], kTopBytes / kRecBytes<W>);
+                const gptr<vf4> src = as_global<vf4>(M.nodesW[width_slot(W)]);
+                for (int k = tid; k < nTop * (kRecBytes<W> / 16); k += kWgSize) sTop[k] = src[k];
             }
         }
         if constexpr (!REC && !CNT) {
@@ -1147,7 +1157,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     if (alive) {
                         bool done;
                         if constexpr (MODE == 1) {
-                            done = ray_step_mixed<SD, CNT, false, WW, false>(
+                            done = ray_step_mixed<SD, CNT, false, WW, false, kWide>(
                                 J, M, a.meshes64[J.mesh], st, sRef, sE, ov, ov64, nullptr, 0, tid, nullptr, w,
                                 cQueries, cFull, a.errK, fell);
                         } else {
@@ -1220,7 +1230,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     bool fell = false;
                     if constexpr (MODE == 1) {
                         const Job<double>& D = a.jobs64[jid];
-                        const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW, TOP>(
+                        const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW, TOP, W>(
                             J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, sTop, nTop, tid, ids, w, cQueries,
                             cFull, a.errK, fell);
                         po_term<double, true>(D, PD, dre, dimg);
@@ -1473,34 +1483,44 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
     if (threadIdx.x == 0) atomicAdd(a.dfreq + lo, red[0] + red[1] + red[2] + red[3]);
 }
 
-template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW>
+template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW, bool TOP = false,
+          int W = kWide>
 hipError_t launch_one(const TraceArgs<R>& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST, POOL, WW>), dim3(grid), dim3(kWgSize), 0, s, a);
+    hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST, POOL, WW, TOP, W>), dim3(grid), dim3(kWgSize), 0,
+                       s, a);
     return hipGetLastError();
 }
 
-template <typename R, int MODE, int SD, bool REC, bool CNT, bool PERSIST, bool POOL, bool WW, bool TOP>
-hipError_t launch_top(const TraceArgs<R>& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((trace_kernel<R, MODE, SD, REC, CNT, PERSIST, POOL, WW, TOP>), dim3(grid), dim3(kWgSize), 0, s,
-                       a);
-    return hipGetLastError();
+// Timed mixed-mode launches (one ray per lane) on the W-wide tree; top:
+// stage the tree's top levels in LDS.
+template <int SD, int W>
+hipError_t launch_mixed_w(const TraceArgs<float>& a, bool persist, bool ww, bool top, int grid, hipStream_t s) {
+    if (top) {
+        if (ww)
+            return persist ? launch_one<float, 1, SD, false, false, true, false, true, true, W>(a, grid, s)
+                           : launch_one<float, 1, SD, false, false, false, false, true, true, W>(a, grid, s);
+        return persist ? launch_one<float, 1, SD, false, false, true, false, false, true, W>(a, grid, s)
+                       : launch_one<float, 1, SD, false, false, false, false, false, true, W>(a, grid, s);
+    }
+    if (ww)
+        return persist ? launch_one<float, 1, SD, false, false, true, false, true, false, W>(a, grid, s)
+                       : launch_one<float, 1, SD, false, false, false, false, true, false, W>(a, grid, s);
+    return persist ? launch_one<float, 1, SD, false, false, true, false, false, false, W>(a, grid, s)
+                   : launch_one<float, 1, SD, false, false, false, false, false, false, W>(a, grid, s);
 }
 
-// Record and counting runs use the while-while shape only (their outputs
-// do not depend on the shape); top: stage the tree's top levels in LDS
-// (mixed mode, one ray per lane).
+// Record and counting runs use the while-while shape and the default node
+// width only (their outputs do not depend on either); width: node width of
+// the mixed mode's tree in timed runs without the ray pool.
 template <typename R, int MODE, int SD, bool POOL>
-hipError_t launch_pool(const TraceArgs<R>& a, bool cnt, bool persist, bool ww, bool top, int grid, hipStream_t s) {
+hipError_t launch_pool(const TraceArgs<R>& a, bool cnt, bool persist, bool ww, bool top, int width, int grid,
+                       hipStream_t s) {
     if (cnt) return persist ? launch_one<R, MODE, SD, false, true, true, POOL, true>(a, grid, s)
                             : launch_one<R, MODE, SD, false, true, false, POOL, true>(a, grid, s);
     if constexpr (MODE == 1 && !POOL) {
-        if (top) {
-            if (ww)
-                return persist ? launch_top<R, MODE, SD, false, false, true, POOL, true, true>(a, grid, s)
-                               : launch_top<R, MODE, SD, false, false, false, POOL, true, true>(a, grid, s);
-            return persist ? launch_top<R, MODE, SD, false, false, true, POOL, false, true>(a, grid, s)
-                           : launch_top<R, MODE, SD, false, false, false, POOL, false, true>(a, grid, s);
-        }
+        if (width == 2) return launch_mixed_w<SD, 2>(a, persist, ww, top, grid, s);
+        if (width == 8) return launch_mixed_w<SD, 8>(a, persist, ww, top, grid, s);
+        return launch_mixed_w<SD, kWide>(a, persist, ww, top, grid, s);
     }
     if (ww)
         return persist ? launch_one<R, MODE, SD, false, false, true, POOL, true>(a, grid, s)
@@ -1510,37 +1530,38 @@ hipError_t launch_pool(const TraceArgs<R>& a, bool cnt, bool persist, bool ww, b
 }
 
 template <typename R, int MODE, int SD>
-hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, bool pool, bool ww, bool top, int grid,
-                     hipStream_t s) {
+hipError_t launch_sd(const TraceArgs<R>& a, bool rec, bool cnt, bool persist, bool pool, bool ww, bool top, int width,
+                     int grid, hipStream_t s) {
     if (rec) return launch_one<R, MODE, SD, true, false, false, false, true>(a, grid, s);
-    return pool ? launch_pool<R, MODE, SD, true>(a, cnt, persist, ww, false, grid, s)
-                : launch_pool<R, MODE, SD, false>(a, cnt, persist, ww, top, grid, s);
+    return pool ? launch_pool<R, MODE, SD, true>(a, cnt, persist, ww, false, kWide, grid, s)
+                : launch_pool<R, MODE, SD, false>(a, cnt, persist, ww, top, width, grid, s);
 }
 
 }  // namespace
 
 // Host entry used by the driver. sd: traversal stack entries kept in LDS
 // per ray (8, 16 or 32); deeper entries spill to scratch. ww: while-while
-// traversal loop (else if-if).
+// traversal loop (else if-if). width: node width (2, 4 or 8) of the mixed
+// mode's tree in timed runs without the ray pool.
 template <typename R>
 hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool cnt, bool persist, bool pool, bool ww,
-                        bool top, int grid, hipStream_t s) {
+                        bool top, int width, int grid, hipStream_t s) {
     if constexpr (std::is_same_v<R, float>) {
         if (mode == 1) {
-            if (sd <= 8) return launch_sd<float, 1, 8>(a, rec, cnt, persist, pool, ww, top, grid, s);
-            if (sd <= 16) return launch_sd<float, 1, 16>(a, rec, cnt, persist, pool, ww, top, grid, s);
-            return launch_sd<float, 1, 32>(a, rec, cnt, persist, pool, ww, top, grid, s);
+            if (sd <= 8) return launch_sd<float, 1, 8>(a, rec, cnt, persist, pool, ww, top, width, grid, s);
+            if (sd <= 16) return launch_sd<float, 1, 16>(a, rec, cnt, persist, pool, ww, top, width, grid, s);
+            return launch_sd<float, 1, 32>(a, rec, cnt, persist, pool, ww, top, width, grid, s);
         }
     }
-    if (sd <= 8) return launch_sd<R, 0, 8>(a, rec, cnt, persist, pool, ww, false, grid, s);
-    if (sd <= 16) return launch_sd<R, 0, 16>(a, rec, cnt, persist, pool, ww, false, grid, s);
-    return launch_sd<R, 0, 32>(a, rec, cnt, persist, pool, ww, false, grid, s);
+    if (sd <= 8) return launch_sd<R, 0, 8>(a, rec, cnt, persist, pool, ww, false, kWide, grid, s);
+    if (sd <= 16) return launch_sd<R, 0, 16>(a, rec, cnt, persist, pool, ww, false, kWide, grid, s);
+    return launch_sd<R, 0, 32>(a, rec, cnt, persist, pool, ww, false, kWide, grid, s);
 }
 
 template hipError_t launch_trace<float>(const TraceArgs<float>&, int, int, bool, bool, bool, bool, bool, bool, int,
-                                        hipStream_t);
+                                        int, hipStream_t);
 template hipError_t launch_trace<double>(const TraceArgs<double>&, int, int, bool, bool, bool, bool, bool, bool, int,
-                                         hipStream_t);
+                                         int, hipStream_t);
 
 template <typename R>
 hipError_t launch_finalize(const Job<R>* jobs, int nJobs, int mode, const void* partials, double* jobOut,
@@ -1572,30 +1593,35 @@ hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <typename R, int MODE, int SD> int occ_one(bool pool, bool top) {
+template <int SD, int W> const void* mixed_fn(bool top) {
+    return top ? reinterpret_cast<const void*>(&trace_kernel<float, 1, SD, false, false, true, false, true, true, W>)
+               : reinterpret_cast<const void*>(&trace_kernel<float, 1, SD, false, false, true, false, true, false, W>);
+}
+
+template <typename R, int MODE, int SD> int occ_one(bool pool, bool top, int width) {
     int n = 0;
     const void* f = pool ? reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, true, true>)
                          : reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, false, true>);
     if constexpr (MODE == 1) {
-        if (top && !pool)
-            f = reinterpret_cast<const void*>(&trace_kernel<R, MODE, SD, false, false, true, false, true, true>);
+        if (!pool) f = width == 2 ? mixed_fn<SD, 2>(top) : (width == 8 ? mixed_fn<SD, 8>(top) : mixed_fn<SD, kWide>(top));
     }
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, kWgSize, 0);
     return e == hipSuccess ? n : 1;
 }
 
 // Resident workgroups per CU of the persistent kernel variant.
-template <typename R> int trace_occupancy(int mode, int sd, bool pool, bool top) {
+template <typename R> int trace_occupancy(int mode, int sd, bool pool, bool top, int width) {
     if constexpr (std::is_same_v<R, float>) {
         if (mode == 1)
-            return sd <= 8 ? occ_one<float, 1, 8>(pool, top)
-                           : (sd <= 16 ? occ_one<float, 1, 16>(pool, top) : occ_one<float, 1, 32>(pool, top));
+            return sd <= 8 ? occ_one<float, 1, 8>(pool, top, width)
+                           : (sd <= 16 ? occ_one<float, 1, 16>(pool, top, width)
+                                       : occ_one<float, 1, 32>(pool, top, width));
     }
-    return sd <= 8 ? occ_one<R, 0, 8>(pool, false)
-                   : (sd <= 16 ? occ_one<R, 0, 16>(pool, false) : occ_one<R, 0, 32>(pool, false));
+    return sd <= 8 ? occ_one<R, 0, 8>(pool, false, kWide)
+                   : (sd <= 16 ? occ_one<R, 0, 16>(pool, false, kWide) : occ_one<R, 0, 32>(pool, false, kWide));
 }
-template int trace_occupancy<float>(int, int, bool, bool);
-template int trace_occupancy<double>(int, int, bool, bool);
+template int trace_occupancy<float>(int, int, bool, bool, int);
+template int trace_occupancy<double>(int, int, bool, bool, int);
 
 // ------------------------------------------------ fixed scratch memory
 // Device storage of the allocation-free sweep (sbr_sweep_noalloc): its job

@@ -453,6 +453,16 @@ def test_tuning_variants(ctx, tri_duct, prec):
             pl.wait()
             z, _, _ = pl.results()
             assert np.array_equal(z, z0), (top, trav)
+    # the mixed mode's binary and eight-wide trees (ignored elsewhere); the
+    # eight-wide tree's top 3 levels do not all fit the LDS copy
+    for width in (2, 8, 4):
+        for top in (0, 3):
+            _step(f"precision {prec} batched node_width={width} top_levels={top}")
+            pl.tune(node_width=width, top_levels=top, traversal=1 if top else 2, tile_rays=32)
+            pl.run()
+            pl.wait()
+            z, _, _ = pl.results()
+            assert np.array_equal(z, z0), (width, top)
     pl.close()
     pa = ctx.plan(jobs, prec, S.SBR_PER_ANGLE)
     for pool in (1, 2):
@@ -464,6 +474,13 @@ def test_tuning_variants(ctx, tri_duct, prec):
             pa.wait()
             z, _, _ = pa.results()
             assert np.array_equal(z, z0), (pool, ts, trav)
+    for width in (2, 8):
+        _step(f"precision {prec} per-angle node_width={width}")
+        pa.tune(node_width=width, top_levels=2)
+        pa.run()
+        pa.wait()
+        z, _, _ = pa.results()
+        assert np.array_equal(z, z0), width
     pa.close()
 
 
@@ -480,11 +497,13 @@ def test_autotune(ctx, tri_duct):
         best = pl.autotune(sample_jobs=8 if method == S.SBR_BATCHED else 0, repeats=1)
         print("   ", best, flush=True)
         # launch shapes, the other loop shape, then (mixed, no ray pool) the
-        # top tree levels in LDS at 3 or 4 levels, also with an 8-entry stack
+        # top tree levels in LDS at 3 or 4 levels, also with an 8-entry stack,
+        # then node widths 2 and 8
         base = (36 if method == S.SBR_BATCHED else 24) + 1
-        assert best["candidates"] in (base, base + 2, base + 4)
+        assert best["candidates"] in {base + a + b for a in (0, 2, 4) for b in (0, 2)}
         assert best["tile_rays"] in (16, 32, 64) and best["lds_stack"] in (8, 16, 32) and best["best_ms"] > 0
         assert best["ray_pool"] in (1, 2) and best["traversal"] in (0, 1) and best["top_levels"] in (0, 3, 4)
+        assert best["node_width"] in (0, 2, 8)
         pl.run()
         pl.wait()
         z, _, _ = pl.results()
