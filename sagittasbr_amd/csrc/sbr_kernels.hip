@@ -1027,6 +1027,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     __shared__ ST sRes[POOL ? 2 * kPoolRays : 1];
     __shared__ int64_t sNext;
     __shared__ unsigned sPool;
+    // next 8x8 ray patch of the current tile (timed runs, one ray per lane)
+    __shared__ int sPatch;
     // projected cull boxes of the current tile's job (one per thread)
     __shared__ CullRect sCull[(REC || CNT) ? 1 : kCullBoxes];
     // the top levels of the current mesh's W-wide tree (TOP)
@@ -1036,9 +1038,6 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    // wave -> 8x8 quadrant of a 16x16 ray pass, lane -> ray within the quadrant
-    const int qx = (wave & 1) * 8 + (lane & 7);
-    const int qy = (wave >> 1) * 8 + (lane >> 3);
     unsigned long long cRays = 0, cVisits = 0, cTests = 0, cBounces = 0, cQueries = 0, cUndecided = 0, cFull = 0;
     typename Ov<R>::T ov[(MODE == 1 ? kMaxStack4 : kMaxStack) - SD];
     // mixed mode: private stack of the FP64 queries that answer ambiguous ones
@@ -1056,6 +1055,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     // n + kWgSize and every thread can track the base itself.
     unsigned poolBase = 0;
     if (POOL && tid == 0) sPool = 0;
+    if (tid == 0) sPatch = 0;
     if (PERSIST) {
         if (tid == 0) sNext = static_cast<int64_t>(atomicAdd(a.tileCounter, 1ull));
     }
@@ -1201,21 +1201,41 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 }
             }
         } else
-        // a tile of side ts is covered by (ts/16)^2 passes of 16x16 rays.
-        // Each wavefront sums its 8x8 patch of every pass without waiting for
-        // the others; after the tile the four patch sums of each pass are
-        // added in wavefront order (the order of wg_sum) and stored at the
-        // pass's place in the job's pass grid
+        // a tile of side ts is covered by (ts/16)^2 passes of 16x16 rays,
+        // each of four 8x8 patches (its quadrants). A wavefront sums one patch
+        // at a time (a butterfly over its 64 rays); after the tile the four
+        // patch sums of each pass are added in quadrant order and stored at
+        // the pass's place in the job's pass grid. In timed runs the
+        // wavefronts take the tile's patches one by one from an LDS counter,
+        // so they reach the end of the tile together; record and counting
+        // runs give wavefront w quadrant w of every pass. The sums do not
+        // depend on which wavefront traced a patch.
         {
         const int perSide = ts >> 4;
-        for (int py = 0; py < ts; py += 16) {
-            for (int px = 0; px < ts; px += 16) {
+        constexpr bool kDyn = !REC && !CNT;
+        const int nPatch = perSide * perSide * 4;
+        for (int p = kDyn ? 0 : wave;; p += kDyn ? 0 : 4) {
+            if constexpr (kDyn) {
+                // all 64 lanes add one (the compiler makes that a single LDS
+                // add of 64 whose result lane k sees plus k), so the counter
+                // advances by 64 per pop and lane 0 holds 64 x the patch. A
+                // pop guarded by `lane == 0` here is threaded by the compiler
+                // with the `lane == 0` store at the loop tail into a nested
+                // loop in which lanes 1..63 go round again without a pop and
+                // never leave it.
+                const int got = atomicAdd(&sPatch, 1);
+                p = __builtin_amdgcn_readfirstlane(got) >> 6;
+            }
+            if (p >= nPatch) break;
+            const int pass = p >> 2, quad = p & 3;
+            const int px = (pass % perSide) * 16, py = (pass / perSide) * 16;
+            {
                 AccT acc = AccT::zero();
-                const int i = J.i0 + tx * ts + px + qx;
-                const int j = J.j0 + ty * ts + py + qy;
+                const int i = J.i0 + tx * ts + px + (quad & 1) * 8 + (lane & 7);
+                const int j = J.j0 + ty * ts + py + (quad >> 1) * 8 + (lane >> 3);
                 bool reach = true;
                 if constexpr (!REC && !CNT) {
-                    const int pi = J.i0 + tx * ts + px + (wave & 1) * 8, pj = J.j0 + ty * ts + py + (wave >> 1) * 8;
+                    const int pi = J.i0 + tx * ts + px + (quad & 1) * 8, pj = J.j0 + ty * ts + py + (quad >> 1) * 8;
                     reach = rays_reach_mesh(J, sCull, pi, min(pi + 7, J.nx - 1), pj, min(pj + 7, J.ny - 1), lane);
                 }
                 do {
@@ -1261,10 +1281,13 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                     else acc.add_ray(re, im);
                 } while (false);
                 acc.wave();
-                if (lane == 0) sPass[((py >> 4) * perSide + (px >> 4)) * (kWgSize / 64) + wave] = acc;
+                if (lane == 0) sPass[pass * (kWgSize / 64) + quad] = acc;
             }
         }
         __syncthreads();
+        // every wavefront has taken its last patch of the tile: the counter
+        // restarts for the next one (made visible by the next tile's barriers)
+        if (kDyn && tid == 0) sPatch = 0;
         if (tid < perSide * perSide) {
             AccT r = sPass[tid * (kWgSize / 64)];
 #pragma unroll
