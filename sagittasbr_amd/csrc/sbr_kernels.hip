@@ -528,11 +528,8 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
         int first = code >> 4;
         int cnt = (code & 15) + 1;
         if (CNT) w.tests += cnt;
-        for (int k = 0; k < cnt; ++k) {
-            V3<double> a, b, c;
-            int f;
-            load_tri_mixed(m, first + k, a, b, c, f);
-            const TriRec<double> tr{a, sub(b, a), sub(c, a), f};
+        auto test = [&](const V3<double>& a, const V3<double>& b, const V3<double>& c, int k) {
+            const TriRec<double> tr{a, sub(b, a), sub(c, a), 0};
             double t;
             if (mt_test(tr, o64, d64, 1e20, t)) {
                 if (t < closest) {
@@ -543,6 +540,33 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
                 } else if (t < second) {
                     second = t;
                 }
+            }
+        };
+        if (m.vertsExact) {
+            // FP32 records widened: the next triangle's record is loaded
+            // while this one is tested
+            const gptr<vf4> p = as_global<vf4>(m.tris) + 3 * first;
+            vf4 ra = p[0], rb = p[1], rc = p[2];
+            for (int k = 0; k < cnt; ++k) {
+                vf4 na = ra, nb = rb, nc = rc;
+                if (k + 1 < cnt) {
+                    na = p[3 * k + 3];
+                    nb = p[3 * k + 4];
+                    nc = p[3 * k + 5];
+                }
+                test(V3<double>{double(ra.x), double(ra.y), double(ra.z)},
+                     V3<double>{double(ra.w), double(rb.x), double(rb.y)},
+                     V3<double>{double(rb.z), double(rb.w), double(rc.x)}, k);
+                ra = na;
+                rb = nb;
+                rc = nc;
+            }
+        } else {
+            for (int k = 0; k < cnt; ++k) {
+                V3<double> a, b, c;
+                int f;
+                load_tri_raw64(m.tris64, first + k, a, b, c, f);
+                test(a, b, c, k);
             }
         }
     };
