@@ -545,18 +545,23 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
         if (m.vertsExact) {
             // FP32 records widened: the next triangle's record is loaded
             // while this one is tested
+            // (the nine vertex floats of the 48-byte record: two 16-byte loads
+            // and one 4-byte load)
             const gptr<vf4> p = as_global<vf4>(m.tris) + 3 * first;
-            vf4 ra = p[0], rb = p[1], rc = p[2];
+            const gptr<float> pf = as_global<float>(m.tris) + 12 * first;
+            vf4 ra = p[0], rb = p[1];
+            float rc = pf[8];
             for (int k = 0; k < cnt; ++k) {
-                vf4 na = ra, nb = rb, nc = rc;
+                vf4 na = ra, nb = rb;
+                float nc = rc;
                 if (k + 1 < cnt) {
                     na = p[3 * k + 3];
                     nb = p[3 * k + 4];
-                    nc = p[3 * k + 5];
+                    nc = pf[12 * k + 20];
                 }
                 test(V3<double>{double(ra.x), double(ra.y), double(ra.z)},
                      V3<double>{double(ra.w), double(rb.x), double(rb.y)},
-                     V3<double>{double(rb.z), double(rb.w), double(rc.x)}, k);
+                     V3<double>{double(rb.z), double(rb.w), double(rc)}, k);
                 ra = na;
                 rb = nb;
                 rc = nc;
