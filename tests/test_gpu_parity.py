@@ -576,3 +576,25 @@ def test_plan_outliving_context_fails_cleanly():
     with pytest.raises(S.SbrError):
         pl.run()
     pl.close()
+
+
+def test_bad_tuning_is_refused_and_leaves_the_plan_unchanged(ctx, tri_duct):
+    v, f, mesh, s32, s64 = tri_duct
+    p = S.SweepParams.make(**dict(C2S, theta_samples=2, phi_samples=3))
+    jobs = S.sweep_jobs(p, mesh, S.SBR_MIXED)
+    pl = ctx.plan(jobs, S.SBR_MIXED, S.SBR_BATCHED)
+    pl.tune(tile_rays=32, top_levels=3)
+    pl.run()
+    pl.wait()
+    z0, _, _ = pl.results()
+    bad = [dict(node_width=3), dict(node_width=16), dict(top_levels=5), dict(top_levels=-1), dict(tile_rays=48),
+           dict(traversal=3), dict(ray_pool=3), dict(lds_stack=-8), dict(wg_per_cu=-1)]
+    for kw in bad:
+        _step(f"refused tuning {kw}")
+        with pytest.raises(S.SbrError, match="must be|negative"):
+            pl.tune(**kw)
+        pl.run()
+        pl.wait()
+        z, _, _ = pl.results()
+        assert np.array_equal(z, z0), kw
+    pl.close()
