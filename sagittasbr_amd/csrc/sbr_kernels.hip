@@ -366,7 +366,7 @@ template <typename R, int SD> struct LaneStack {
 // Closest hit (rtKernels.cu:63-100). Returns the BVH position of the hit
 // triangle or -1. The host guarantees tree depth <= kMaxDepth so the stack
 // stays below kMaxStack entries.
-template <typename R, int SD, bool CNT, bool WW>
+template <typename R, int SD, bool CNT, bool WW, bool PF = true>
 __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V3<R> inv, R& tHit, int* sRef, R* sE,
                                            typename Ov<R>::T* ov, int tid, Work& w) {
     const R* nodes = static_cast<const R*>(m.nodes);
@@ -401,12 +401,42 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
         int first = code >> 4;
         int cnt = (code & 15) + 1;
         if (CNT) w.tests += cnt;
-        for (int k = 0; k < cnt; ++k) {
-            TriRec<R> tr = load_tri(tris, first + k);
-            R t;
-            if (mt_test(tr, o, d, closest, t)) {
-                closest = t;
-                hit = first + k;
+        if constexpr (std::is_same_v<R, float> && PF) {
+            // the next triangle's nine vertex floats are loaded while this
+            // one is tested (two 16-byte loads and one 4-byte load); the
+            // edges are formed as load_tri forms them. PF false: the kernel
+            // holds too many registers for the nine (the ray pool's)
+            const gptr<vf4> p = as_global<vf4>(tris) + 3 * first;
+            const gptr<float> pf = as_global<float>(tris) + 12 * first;
+            vf4 ra = p[0], rb = p[1];
+            float rc = pf[8];
+            for (int k = 0; k < cnt; ++k) {
+                vf4 na = ra, nb = rb;
+                float nc = rc;
+                if (k + 1 < cnt) {
+                    na = p[3 * k + 3];
+                    nb = p[3 * k + 4];
+                    nc = pf[12 * k + 20];
+                }
+                const V3<float> v0{ra.x, ra.y, ra.z}, v1{ra.w, rb.x, rb.y}, v2{rb.z, rb.w, rc};
+                const TriRec<float> tr{v0, sub(v1, v0), sub(v2, v0), 0};
+                R t;
+                if (mt_test(tr, o, d, closest, t)) {
+                    closest = t;
+                    hit = first + k;
+                }
+                ra = na;
+                rb = nb;
+                rc = nc;
+            }
+        } else {
+            for (int k = 0; k < cnt; ++k) {
+                TriRec<R> tr = load_tri(tris, first + k);
+                R t;
+                if (mt_test(tr, o, d, closest, t)) {
+                    closest = t;
+                    hit = first + k;
+                }
             }
         }
     };
@@ -727,13 +757,13 @@ template <typename R> __device__ __forceinline__ void ray_init(const Job<R>& J, 
 
 // One query of bounce st.k; returns true when the ray has ended (miss or
 // max_bounces reached). Requires st.k < J.maxB.
-template <typename R, int SD, bool CNT, bool REC, bool WW>
+template <typename R, int SD, bool CNT, bool REC, bool WW, bool PF = true>
 __device__ __forceinline__ bool ray_step(const Job<R>& J, const Mesh<R>& M, RayState<R>& st, int* sRef, R* sE,
                                          typename Ov<R>::T* ov, int tid, int32_t* ids, Work& w,
                                          unsigned long long& queries) {
     V3<R> inv{R(1.0) / st.d.x, R(1.0) / st.d.y, R(1.0) / st.d.z};
     R t;
-    int h = closest_hit<R, SD, CNT, WW>(M, st.o, st.d, inv, t, sRef, sE, ov, tid, w);
+    int h = closest_hit<R, SD, CNT, WW, PF>(M, st.o, st.d, inv, t, sRef, sE, ov, tid, w);
     if (CNT) queries++;
     if (h < 0) return true;
     TriRec<R> tr = load_tri(static_cast<const R*>(M.tris), h);
@@ -1190,7 +1220,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                                 J, M, a.meshes64[J.mesh], st, sRef, sE, ov, ov64, nullptr, 0, tid, nullptr, w,
                                 cQueries, cFull, a.errK, fell);
                         } else {
-                            done = ray_step<R, SD, CNT, false, WW>(J, M, st, sRef, sE, ov, tid, nullptr, w,
+                            done = ray_step<R, SD, CNT, false, WW, false>(J, M, st, sRef, sE, ov, tid, nullptr, w,
                                                                    cQueries);
                         }
                         if (done) {
