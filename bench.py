@@ -409,8 +409,9 @@ def sweep_workload(args):
         apply_tuning(p64, dict(tuning, use_graph=0))
         fp64_ms, _ = timed([p64], 1, 0)
         _, _, db64 = p64.results()
-        log("FP64 counting run (reference traversal counters)")
+        log("FP64 counting runs (reference traversal counters: all rays, then the rays a timed run traces)")
         cnt = p64.count()
+        cnt_tr = p64.count(traced=True)
         p64.close()
         log("mixed counting run")
         own = plans[0].count()
@@ -427,11 +428,14 @@ def sweep_workload(args):
         pa_ms, _ = timed([pa], 1, 0)
         pa.close()
         per_angle_rate = n0 / (pa_ms / 1000.0)
-        alg_ops = OPS_BOX * cnt["node_visits"] + OPS_TRI * cnt["tri_tests"] + OPS_QUERY * cnt["queries"]
+        # the work of the rays the timed launch traces (the coverage masks
+        # skip patches none of whose rays can reach the mesh)
+        alg_ops = OPS_BOX * cnt_tr["node_visits"] + OPS_TRI * cnt_tr["tri_tests"] + OPS_QUERY * cnt_tr["queries"]
+        alg_all = OPS_BOX * cnt["node_visits"] + OPS_TRI * cnt["tri_tests"] + OPS_QUERY * cnt["queries"]
         achieved = alg_ops / (k0_ms * 1e-3) / 1e12
         out.update({
             "ray_bounces_per_s": round(cnt["ray_bounces"] / (k0_ms * 1e-3), 1),
-            "bvh_queries_per_s": round(cnt["queries"] / (k0_ms * 1e-3), 1),
+            "bvh_queries_per_s": round(cnt_tr["queries"] / (k0_ms * 1e-3), 1),
             "per_angle_fp32_angles_per_s": round(per_angle_rate, 2), "per_angle_fp32_tuning": t_pa,
             "batched_over_per_angle": round(value / per_angle_rate, 3),
             "batched_fp32_angles_per_s": round(n0 / (fp32_ms / 1000.0), 2), "batched_fp32_tuning": t32,
@@ -440,6 +444,7 @@ def sweep_workload(args):
             "accuracy_vs_fp64_db": db_err(db[0::ns], db64),
             "fp32_accuracy_vs_fp64_db": db_err(db32, db64),
             "counters_fp64_traversal_slice0": cnt,
+            "counters_fp64_traversal_traced_slice0": cnt_tr,
             "undecided_rays_slice0": own["retraced"], "full_fp64_queries_slice0": own["fp64_queries"],
             "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": VALU_PEAK_TOPS,
                          "unit": "Tops/s", "frac": round(achieved / VALU_PEAK_TOPS, 5), "traffic": None,
@@ -447,7 +452,9 @@ def sweep_workload(args):
                          "algorithmic_ops": int(alg_ops),
                          "ops_model": f"{OPS_BOX} per box test x node visits + {OPS_TRI} per triangle test + "
                                       f"{OPS_QUERY} per query, reference traversal counts (FP64 counting run "
-                                      f"over the same launch's jobs)"},
+                                      f"over the rays a timed run of the same launch's jobs traces)",
+                         "algorithmic_ops_all_rays": int(alg_all),
+                         "frac_all_rays": round(alg_all / (k0_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS, 5)},
         })
     plans[0].close()
     if not args.no_pmc and "roofline" in out:

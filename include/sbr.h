@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SBR_ABI_VERSION 4
+#define SBR_ABI_VERSION 5
 
 typedef struct sbr_ctx sbr_ctx;
 typedef struct sbr_plan sbr_plan;
@@ -210,8 +210,15 @@ int sbr_plan_last_ms(sbr_plan* plan, double* ms);
 /* Per-job complex sum (2 doubles per job), sigma = 4*pi*|sum|^2 and dBsm
  * (src/app/sweep.cu:228-231); any output may be NULL. */
 int sbr_plan_results(sbr_plan* plan, double* reim, double* sigma, double* dbsm);
-/* Run once with work counters enabled (slower; not for timing). */
+/* Run once with work counters enabled (slower; not for timing). Every ray of
+ * each job's culled rectangle is traced, as the reference traces every ray
+ * of its grid (src/cuda/rtKernels.cu:105-158). */
 int sbr_plan_count(sbr_plan* plan, sbr_counters* out);
+/* The same counters over the rays a timed run of the plan traces: tiles and
+ * 8 x 8 ray patches that the coverage masks rule out (no ray of theirs can
+ * reach a triangle) are skipped as in timed runs. Batched plans only. No
+ * counterpart in the reference; it gives the work one timed launch does. */
+int sbr_plan_count_traced(sbr_plan* plan, sbr_counters* out);
 int sbr_plan_destroy(sbr_plan* plan);
 
 /* Autotuner (no counterpart in the reference, whose launch shape is fixed at

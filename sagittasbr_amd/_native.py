@@ -91,7 +91,7 @@ class Job(C.Structure):
     ]
 
 
-ABI_VERSION = 4  # SBR_ABI_VERSION of the structures below
+ABI_VERSION = 5  # SBR_ABI_VERSION of the structures below
 
 
 class Tuning(C.Structure):
@@ -179,6 +179,7 @@ def lib():
         "sbr_plan_last_ms": (C.c_int, [_P, f64p]),
         "sbr_plan_results": (C.c_int, [_P, _P, _P, _P]),
         "sbr_plan_count": (C.c_int, [_P, C.POINTER(Counters)]),
+        "sbr_plan_count_traced": (C.c_int, [_P, C.POINTER(Counters)]),
         "sbr_plan_destroy": (C.c_int, [_P]),
         "sbr_sweep": (C.c_int, [_P, C.c_int, C.POINTER(SweepParams), C.c_int, C.c_int, _P, _P, _P]),
         "sbr_sweep_noalloc": (C.c_int, [_P, C.c_int, C.POINTER(SweepParams), C.c_int, _P, _P, _P]),
@@ -328,9 +329,13 @@ class Plan:
         _check(lib().sbr_plan_results(self.h, _ptr(reim), _ptr(sig), _ptr(db)), self.ctx.h)
         return reim[0::2] + 1j * reim[1::2], sig, db
 
-    def count(self):
+    def count(self, traced=False):
+        """Work counters of one run: over every ray of each job's culled
+        rectangle, or (traced=True, batched plans) over the rays a timed run
+        traces."""
         c = Counters()
-        _check(lib().sbr_plan_count(self.h, C.byref(c)), self.ctx.h)
+        fn = lib().sbr_plan_count_traced if traced else lib().sbr_plan_count
+        _check(fn(self.h, C.byref(c)), self.ctx.h)
         return c.as_dict()
 
     def close(self):

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <queue>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -520,6 +521,51 @@ template <typename R> std::vector<float> cull_boxes(const Bvh<R>& b, int maxBoxe
     return out;
 }
 
+template <typename R> std::vector<float> cover_boxes(const Bvh<R>& b, int maxBoxes) {
+    auto area = [&](int i) {
+        const RefNode<R>& n = b.nodes[i];
+        const double x = double(n.hi[0]) - n.lo[0], y = double(n.hi[1]) - n.lo[1], z = double(n.hi[2]) - n.lo[2];
+        return x * y + y * z + z * x;
+    };
+    // open internal nodes by area (largest first); leaves are final
+    std::priority_queue<std::pair<double, int>> open;
+    std::vector<int> done;
+    int count = 1;
+    if (b.nodes[b.root].triCount == 0) open.push({area(b.root), b.root});
+    else done.push_back(b.root);
+    while (!open.empty() && count + 1 <= maxBoxes) {
+        const int n = open.top().second;
+        open.pop();
+        ++count;
+        for (int c : {b.nodes[n].left, b.nodes[n].right}) {
+            if (b.nodes[c].triCount == 0) open.push({area(c), c});
+            else done.push_back(c);
+        }
+    }
+    while (!open.empty()) {
+        done.push_back(open.top().second);
+        open.pop();
+    }
+    std::vector<float> out;
+    out.reserve(done.size() * 6);
+    for (int n : done) {
+        const RefNode<R>& nd = b.nodes[n];
+        for (int k = 0; k < 3; ++k) {
+            float lo = static_cast<float>(nd.lo[k]);
+            if (double(lo) > double(nd.lo[k])) lo = std::nextafter(lo, -INFINITY);
+            out.push_back(lo);
+        }
+        for (int k = 0; k < 3; ++k) {
+            float hi = static_cast<float>(nd.hi[k]);
+            if (double(hi) < double(nd.hi[k])) hi = std::nextafter(hi, INFINITY);
+            out.push_back(hi);
+        }
+    }
+    return out;
+}
+
+template std::vector<float> cover_boxes<float>(const Bvh<float>&, int);
+template std::vector<float> cover_boxes<double>(const Bvh<double>&, int);
 template std::vector<float> cull_boxes<float>(const Bvh<float>&, int);
 template std::vector<float> cull_boxes<double>(const Bvh<double>&, int);
 template Bvh<float> build_bvh<float>(const std::vector<float>&, const BuildOptions&);

@@ -84,6 +84,9 @@ bool to_gpu_layout_wide(const Bvh<float>& b, int width, GpuBvhW& out, std::strin
 // frontier reached by opening the largest-area internal node first. Written
 // as FP32 lo.xyz hi.xyz, rounded outwards.
 template <typename R> std::vector<float> cull_boxes(const Bvh<R>& b, int maxBoxes);
+// The same cut for large maxBoxes (a heap instead of a scan per step; ties
+// between equal areas may open in another order than cull_boxes opens them).
+template <typename R> std::vector<float> cover_boxes(const Bvh<R>& b, int maxBoxes);
 
 inline int encode_leaf(int firstTri, int count) { return ~((firstTri << 4) | (count - 1)); }
 

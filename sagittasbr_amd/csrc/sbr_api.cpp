@@ -29,6 +29,9 @@ hipError_t launch_trace(const TraceArgs<R>& a, int mode, int sd, bool rec, bool 
                         bool top, int width, int grid, hipStream_t s);
 hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s);
 template <typename R>
+hipError_t launch_coverage(const Job<R>* jobs, const Mesh<R>* meshes, int nJobs, int maxChunks,
+                           unsigned long long* mask, hipStream_t s);
+template <typename R>
 hipError_t launch_finalize(const Job<R>* jobs, int nJobs, int mode, const void* partials, double* jobOut,
                            hipStream_t s);
 size_t partial_bytes(bool fp64, int mode);
@@ -98,6 +101,8 @@ template <typename R> struct PrecMesh {
     bool builtW[3] = {false, false, false};
     DevBuf cull;    // boxes of the per-tile emptiness test
     int nCull = 0;
+    DevBuf cov;     // boxes of the batched runs' coverage masks
+    int nCov = 0;
     double buildMs = 0;
 };
 
@@ -148,6 +153,14 @@ struct sbr_plan {
     DevBuf dJobs, dPartials, dJobOut, dCounter, dCounters;
     // mixed mode
     DevBuf dJobs64, dRecFlag;
+    // batched runs: the jobs' ray-patch coverage masks (maskWords 64-bit
+    // words), rebuilt by every run from the coverage boxes of its meshes in
+    // LDS chunks (at most maxChunks per job)
+    DevBuf dMask;
+    int64_t maskWords = 0;
+    int maxChunks = 0;
+    // a counting run skips what the coverage masks rule out (sbr_plan_count_traced)
+    bool countTraced = false;
     std::vector<double> out;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
@@ -279,6 +292,18 @@ template <typename R> int ensure_built(sbr_ctx* ctx, MeshEntry& m) {
         pm.nCull = int(cb.size() / 6);
         HIP_TRY(&ctx->err, pm.cull.ensure(cb.size() * sizeof(float)));
         HIP_TRY(&ctx->err, hipMemcpy(pm.cull.p, cb.data(), cb.size() * sizeof(float), hipMemcpyHostToDevice));
+        std::vector<float> vb = cover_boxes<R>(pm.bvh, kCovBoxes);
+        if constexpr (std::is_same_v<R, float>) {
+            if (!pm.vertsExact)
+                for (size_t b = 0; b < vb.size() / 6; ++b)
+                    for (int k = 0; k < 3; ++k) {
+                        vb[6 * b + k] = widen_ulps(vb[6 * b + k], -kWidenUlps);
+                        vb[6 * b + 3 + k] = widen_ulps(vb[6 * b + 3 + k], kWidenUlps);
+                    }
+        }
+        pm.nCov = int(vb.size() / 6);
+        HIP_TRY(&ctx->err, pm.cov.ensure(vb.size() * sizeof(float)));
+        HIP_TRY(&ctx->err, hipMemcpy(pm.cov.p, vb.data(), vb.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     pm.built = true;
     pm.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -305,6 +330,8 @@ template <typename R> int upload_table(sbr_ctx* ctx) {
             mm.tris64 = pm.tris64.p;
             mm.cull = static_cast<const float*>(pm.cull.p);
             mm.nCull = pm.nCull;
+            mm.cov = static_cast<const float*>(pm.cov.p);
+            mm.nCov = pm.nCov;
             if constexpr (std::is_same_v<R, float>) {
                 for (int w = 0; w < 3; ++w) {
                     mm.nodesW[w] = pm.builtW[w] ? pm.nodesW[w].p : nullptr;
@@ -494,6 +521,7 @@ struct DevPlan {
     const int* chunkJob = nullptr;
     void* jobs64 = nullptr;
     int8_t* recFlag = nullptr;
+    unsigned long long* mask = nullptr;  // batched runs: coverage masks (null: the kCullBoxes test)
 };
 
 // Fills hp and the plan's tile bookkeeping. noAlloc: fail instead of
@@ -509,8 +537,8 @@ template <typename R> int plan_host(sbr_plan* pl, bool record, bool noAlloc, Hos
     pl->nTiles.assign(pl->nJobs, 0);
     int64_t base = 0;
     int64_t recRays = 0, recIds = 0;
-    int64_t rays = 0, passes = 0;
-    int maxDepth = 0;
+    int64_t rays = 0, passes = 0, maskWords = 0;
+    int maxDepth = 0, maxChunks = 0;
     const int ts = pl->tileSide;
     for (int i = 0; i < pl->nJobs; ++i) {
         const sbr_job& j = pl->jobs[i];
@@ -612,6 +640,12 @@ template <typename R> int plan_host(sbr_plan* pl, bool record, bool noAlloc, Hos
         J.tilesY = any ? (j1 - j0 + ts) / ts : 0;
         J.nTiles = J.tilesX * J.tilesY;
         J.tileBase = base;
+        // coverage mask: 64 x 64-ray blocks over the job's tiles
+        J.maskX = (J.tilesX * ts + 63) / 64;
+        J.maskY = (J.tilesY * ts + 63) / 64;
+        J.maskBase = maskWords;
+        maskWords += int64_t(J.maskX) * J.maskY;
+        maxChunks = std::max(maxChunks, cov_chunks(J.maskX, J.maskY));
         J.passX = any ? (i1 - i0 + 16) / 16 : 0;
         J.passY = any ? (j1 - j0 + 16) / 16 : 0;
         J.passBase = passes;
@@ -649,6 +683,8 @@ template <typename R> int plan_host(sbr_plan* pl, bool record, bool noAlloc, Hos
         }
     }
     pl->maxDepth = maxDepth;
+    pl->maxChunks = maxChunks;
+    pl->maskWords = maskWords;
     hp.passes = passes;
     return 0;
 }
@@ -671,6 +707,8 @@ template <typename R> int plan_prepare(sbr_plan* pl, bool record) {
     HIP_TRY(&ctx->err, pl->dJobOut.ensure(sizeof(double) * 2 * std::max(pl->nJobs, 1)));
     HIP_TRY(&ctx->err, pl->dCounter.ensure(sizeof(unsigned long long)));
     HIP_TRY(&ctx->err, pl->dCounters.ensure(sizeof(unsigned long long) * 16));
+    if (pl->method == SBR_BATCHED && !record)
+        HIP_TRY(&ctx->err, pl->dMask.ensure(sizeof(unsigned long long) * std::max<int64_t>(pl->maskWords, 1)));
     if (mixed) {
         HIP_TRY(&ctx->err, pl->dJobs64.ensure(sizeof(Job<double>) * std::max(pl->nJobs, 1)));
         HIP_TRY(&ctx->err,
@@ -689,6 +727,7 @@ DevPlan dev_of(sbr_plan* pl) {
     d.chunkJob = static_cast<const int*>(pl->dChunkJob.p);
     d.jobs64 = pl->dJobs64.p;
     d.recFlag = static_cast<int8_t*>(pl->dRecFlag.p);
+    d.mask = pl->method == SBR_BATCHED ? static_cast<unsigned long long*>(pl->dMask.p) : nullptr;
     return d;
 }
 
@@ -719,7 +758,7 @@ void release_plan_device(sbr_plan* pl) {
     if (pl->ev1) (void)hipEventDestroy(pl->ev1);
     pl->ev0 = pl->ev1 = nullptr;
     for (DevBuf* b : {&pl->dJobs, &pl->dPartials, &pl->dJobOut, &pl->dCounter, &pl->dCounters, &pl->dChunkJob,
-                      &pl->dJobs64, &pl->dRecFlag})
+                      &pl->dJobs64, &pl->dRecFlag, &pl->dMask})
         b->release();
     pl->ctx = nullptr;
 }
@@ -764,9 +803,15 @@ template <typename R> int plan_enqueue(sbr_plan* pl, const DevPlan& d, bool reco
         a.errK = kErrSafety;
         a.recFlag = d.recFlag;
     }
+    // batched runs cull against the jobs' coverage masks, built first
+    const bool masked = pl->method == SBR_BATCHED && !record && d.mask && pl->totalTiles > 0 &&
+                        (!count || pl->countTraced);
+    a.mask = masked ? d.mask : nullptr;
+    a.countCulled = count && masked ? 1 : 0;
     hipStream_t s = ctx->stream;
     HIP_TRY(&ctx->err, launch_zero(reinterpret_cast<unsigned*>(d.tileCounter), 2, s));
     HIP_TRY(&ctx->err, launch_zero(reinterpret_cast<unsigned*>(d.counters), 32, s));
+    if (masked) HIP_TRY(&ctx->err, launch_coverage<R>(a.jobs, a.meshes, pl->nJobs, pl->maxChunks, d.mask, s));
     // the timed region covers the trace launches only
     if (timed) HIP_TRY(&ctx->err, hipEventRecord(pl->ev0, s));
     if (pl->totalTiles > 0) {
@@ -1368,12 +1413,17 @@ int sbr_plan_results(sbr_plan* pl, double* reim, double* sigma, double* dbsm) {
     return 0;
 }
 
-int sbr_plan_count(sbr_plan* pl, sbr_counters* out) {
+static int plan_count(sbr_plan* pl, sbr_counters* out, bool traced) {
     if (!pl || !out) return fail(nullptr, "null argument");
     if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     sbr_ctx* ctx = pl->ctx;
+    if (traced && pl->method != SBR_BATCHED)
+        return fail(&ctx->err, "counting the traced rays needs a batched plan (coverage masks)");
     HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
-    if (dispatch_launch(pl, false, true)) return 1;
+    pl->countTraced = traced;
+    const int rc = dispatch_launch(pl, false, true);
+    pl->countTraced = false;
+    if (rc) return 1;
     unsigned long long c[16] = {0};
     HIP_TRY(&ctx->err, hipMemcpyAsync(c, pl->dCounters.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(&ctx->err, hipStreamSynchronize(ctx->stream));
@@ -1389,6 +1439,8 @@ int sbr_plan_count(sbr_plan* pl, sbr_counters* out) {
     out->fp64_queries = c[6];
     return 0;
 }
+int sbr_plan_count(sbr_plan* pl, sbr_counters* out) { return plan_count(pl, out, false); }
+int sbr_plan_count_traced(sbr_plan* pl, sbr_counters* out) { return plan_count(pl, out, true); }
 
 int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* best, double* best_ms, int* n_tried) {
     if (!pl) return fail(nullptr, "null plan");

@@ -24,6 +24,21 @@ constexpr int max_depth_wide(int w) { return kMaxStack4 / (w - 1); }
 constexpr int width_slot(int w) { return w == 2 ? 0 : (w == 8 ? 2 : 1); }
 constexpr int kEmptySlot = 0x7fffffff;  // unused child slot of a wide record
 constexpr int kCullBoxes = 256;         // boxes of the tile and ray-patch emptiness tests
+// Boxes of the batched runs' ray-patch coverage masks: a cut of the BVH
+// (every triangle lies in one), at most this many
+constexpr int kCovBoxes = 65536;
+// Mask words a workgroup of the coverage kernel builds in LDS (64 KB)
+constexpr int kCovLdsWords = 8192;
+// LDS chunks of a job's coverage mask of maskX x maskY words: rows of at
+// most kCovLdsWords words, as many rows as fit
+inline int cov_chunks(int maskX, int maskY) {
+    if (maskX <= 0 || maskY <= 0) return 0;
+    const int cols = maskX < kCovLdsWords ? maskX : kCovLdsWords;
+    int rows = kCovLdsWords / cols;
+    if (rows < 1) rows = 1;
+    if (rows > maskY) rows = maskY;
+    return ((maskX + cols - 1) / cols) * ((maskY + rows - 1) / rows);
+}
 // Top levels of the wide tree that a workgroup can stage in LDS: at most
 // kTopBytes of records (1 + 4 + 16 + 64 = 85 four-wide records)
 constexpr int kTopLevels = 4;
@@ -52,6 +67,12 @@ template <typename R> struct alignas(16) Job {
     // products and sums grow with the coordinates' size against the grid)
     float cullErrA, cullErrB;
     int64_t tileBase;                // first global tile of this job
+    // batched runs: this job's ray-patch coverage mask, one 64-bit word per
+    // 64 x 64 ray block of its tile rectangle (maskX x maskY blocks from
+    // (i0, j0), row-major from maskBase), bit (y * 8 + x) for the block's
+    // 8 x 8 ray patch (x, y)
+    int64_t maskBase;
+    int maskX, maskY;
     // record runs: this job's first ray in the per-ray arrays and first
     // entry of the hit-face array (jobs one after another)
     int64_t recBase, recIdBase;
@@ -85,6 +106,10 @@ template <typename R> struct alignas(16) Mesh {
     // together hold every triangle, for the per-tile emptiness test
     const float* cull;
     int nCull;
+    // up to kCovBoxes boxes of the same kind (a finer cut of the tree) for
+    // the batched runs' coverage masks
+    const float* cov;
+    int nCov;
     R rootLo[3], rootHi[3];
     // FP32 tree only: the root box for the mixed mode, bounding the FP64
     // triangles (the FP32 root box widened when vertsExact is 0)
@@ -128,6 +153,11 @@ template <typename R> struct TraceArgs {
     int8_t* recFlag;
     // levels of the wide tree staged in LDS (mixed mode, 1 .. kTopLevels)
     int topLevels;
+    // ray-patch coverage masks of the jobs (batched runs; null: tiles and
+    // patches are tested against the kCullBoxes boxes instead); countCulled:
+    // a counting run skips what the masks rule out, as a timed run does
+    const unsigned long long* mask;
+    int countCulled;
 };
 
 // Arguments of the frozen-path adjoint kernel (one launch per job).

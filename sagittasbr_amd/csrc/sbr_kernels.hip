@@ -997,9 +997,7 @@ __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J,
 struct CullRect {
     float amin, amax, bmin, bmax;
 };
-template <typename R> __device__ __forceinline__ CullRect cull_rect_of(const Job<R>& J, const Mesh<R>& M, int box) {
-    if (box >= M.nCull) return CullRect{3.0e38f, -3.0e38f, 3.0e38f, -3.0e38f};
-    gptr<float> c = as_global<float>(M.cull) + 6 * box;
+template <typename R> __device__ __forceinline__ CullRect box_rect(const Job<R>& J, gptr<float> c) {
     float amin = -J.cullA[3], amax = -J.cullA[3], bmin = -J.cullB[3], bmax = -J.cullB[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1012,6 +1010,10 @@ template <typename R> __device__ __forceinline__ CullRect cull_rect_of(const Job
         bmax += fmaxf(pb, qb);
     }
     return CullRect{amin, amax, bmin, bmax};
+}
+template <typename R> __device__ __forceinline__ CullRect cull_rect_of(const Job<R>& J, const Mesh<R>& M, int box) {
+    if (box >= M.nCull) return CullRect{3.0e38f, -3.0e38f, 3.0e38f, -3.0e38f};
+    return box_rect(J, as_global<float>(M.cull) + 6 * box);
 }
 
 // Whether some ray of the ray rectangle [ia, ib] x [ja, jb] of a job can
@@ -1034,6 +1036,88 @@ __device__ __forceinline__ bool rays_reach_mesh(const Job<R>& J, const CullRect*
         reach = reach || (r.amax >= aLo && r.amin <= aHi && r.bmax >= bLo && r.bmin <= bHi);
     }
     return __ballot(reach) != 0ull;
+}
+
+// Coverage masks of a batched run. A job's mask words are built in LDS by
+// one workgroup per chunk of at most kCovLdsWords words (a rectangle of its
+// 64 x 64-ray blocks; blockIdx.x is the chunk, blockIdx.y strides over the
+// jobs), whose threads take the mesh's coverage boxes in turn: a box's
+// rectangle, widened as rays_reach_mesh widens the cull rectangles, becomes
+// the range of rays whose centres it can hold, one more ray on each side
+// for the FP32 rounding of that conversion (far below one ray spacing at
+// any grid size an int ray index allows), clipped to the job's mask region;
+// the 8 x 8 ray patches that range touches in the chunk get their bits set.
+// The chunk's words are then stored, so every word of every job is written
+// once per run and needs no clearing. The coverage boxes are a cut of the
+// BVH, so a patch whose bit stays clear holds no ray that can reach a
+// triangle: its rays contribute exactly zero.
+template <typename R>
+__global__ __launch_bounds__(256) void coverage_kernel(const Job<R>* jobs, const Mesh<R>* meshes, int nJobs,
+                                                       unsigned long long* mask) {
+    __shared__ unsigned long long sMask[kCovLdsWords];
+    const int tid = static_cast<int>(threadIdx.x);
+    for (int jid = blockIdx.y; jid < nJobs; jid += gridDim.y) {
+        const Job<R>& J = jobs[jid];
+        if (J.maskX == 0) continue;  // (uniform: no tiles)
+        // the chunk: cw x ch blocks from block (cx0, cy0)
+        const int cols = min(J.maskX, kCovLdsWords), rows = min(max(kCovLdsWords / cols, 1), J.maskY);
+        const int nCx = (J.maskX + cols - 1) / cols, nCy = (J.maskY + rows - 1) / rows;
+        const int c = static_cast<int>(blockIdx.x);
+        if (c >= nCx * nCy) continue;  // (uniform)
+        const int cx0 = (c % nCx) * cols, cy0 = (c / nCx) * rows;
+        const int cw = min(cols, J.maskX - cx0), ch = min(rows, J.maskY - cy0);
+        for (int k = tid; k < cw * ch; k += 256) sMask[k] = 0ull;
+        __syncthreads();
+        const Mesh<R>& M = meshes[J.mesh];
+        const float ma = 1e-5f + 0.5f / float(J.nx) + J.cullErrA, mb = 1e-5f + 0.5f / float(J.ny) + J.cullErrB;
+        // the chunk's first and last ray column and row (inside the grid)
+        const int iLo = J.i0 + 64 * cx0, iHi = min(J.i0 + 64 * (cx0 + cw), J.nx) - 1;
+        const int jLo = J.j0 + 64 * cy0, jHi = min(J.j0 + 64 * (cy0 + ch), J.ny) - 1;
+        for (int box = tid; box < M.nCov; box += 256) {
+            const CullRect r = box_rect(J, as_global<float>(M.cov) + 6 * box);
+            // (NaN-free: fmaxf/fminf return the non-NaN operand, and an
+            // empty rectangle keeps lo > hi)
+            const float fa0 = fmaxf(floorf((r.amin - ma) * float(J.nx) - 0.5f) - 1.0f, float(iLo));
+            const float fa1 = fminf(ceilf((r.amax + ma) * float(J.nx) - 0.5f) + 1.0f, float(iHi));
+            const float fb0 = fmaxf(floorf((r.bmin - mb) * float(J.ny) - 0.5f) - 1.0f, float(jLo));
+            const float fb1 = fminf(ceilf((r.bmax + mb) * float(J.ny) - 0.5f) + 1.0f, float(jHi));
+            if (!(fa0 <= fa1) || !(fb0 <= fb1)) continue;
+            // patch columns and rows relative to the job's mask region
+            const int gx0 = (int(fa0) - J.i0) >> 3, gx1 = (int(fa1) - J.i0) >> 3;
+            const int gy0 = (int(fb0) - J.j0) >> 3, gy1 = (int(fb1) - J.j0) >> 3;
+            for (int by = gy0 >> 3; by <= gy1 >> 3; ++by) {
+                const int ly0 = max(gy0 - 8 * by, 0), ly1 = min(gy1 - 8 * by, 7);
+                for (int bx = gx0 >> 3; bx <= gx1 >> 3; ++bx) {
+                    const int lx0 = max(gx0 - 8 * bx, 0), lx1 = min(gx1 - 8 * bx, 7);
+                    const unsigned long long row = (0xffull >> (7 - lx1)) & (0xffull << lx0);
+                    unsigned long long bits = 0;
+                    for (int ly = ly0; ly <= ly1; ++ly) bits |= row << (8 * ly);
+                    atomicOr(&sMask[(by - cy0) * cw + (bx - cx0)], bits);
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < cw * ch; k += 256)
+            mask[J.maskBase + int64_t(cy0 + k / cw) * J.maskX + cx0 + k % cw] = sMask[k];
+        // (the next job's chunk reuses the LDS words)
+        __syncthreads();
+    }
+}
+
+// Bits of one tile's 8 x 8 ray patches in its job's coverage mask, bit
+// (y * P + x) for the tile's patch (x, y), P = ts / 8 patches per side (ts
+// = 16, 32 or 64, so the tile lies inside one 64 x 64 block).
+template <typename R>
+__device__ __forceinline__ unsigned long long tile_patch_bits(const Job<R>& J, const unsigned long long* mask, int tx,
+                                                              int ty, int ts) {
+    const int P = ts >> 3;
+    const int gx = tx * P, gy = ty * P;  // the tile's first patch column and row
+    const unsigned long long w = mask[J.maskBase + int64_t(gy >> 3) * J.maskX + (gx >> 3)];
+    if (P == 8) return w;
+    const unsigned long long row = (1ull << P) - 1ull;
+    unsigned long long bits = 0;
+    for (int y = 0; y < P; ++y) bits |= ((w >> (8 * ((gy & 7) + y) + (gx & 7))) & row) << (P * y);
+    return bits;
 }
 
 // Per-ray result slot of the ray-pool variant: the ray's (re, im) term in
@@ -1141,10 +1225,15 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
         const int ts = a.tileSide;
         const int tx = tl % J.tilesX, ty = tl / J.tilesX;
         // tiles and 8x8 ray patches none of whose rays can reach the mesh
-        // are not traced (not in record or counting runs, which account for
-        // every ray). Every wave has passed the previous tile's last barrier,
-        // so the rectangles of its job can be overwritten.
+        // are not traced (not in record runs, which account for every ray,
+        // nor in counting runs unless they count what a timed run traces).
+        // Batched runs read the job's coverage mask; the others test the
+        // kCullBoxes rectangles, into which every wave may write once it has
+        // passed the previous tile's last barrier.
         bool skip = false;
+        const bool useMask = !REC && a.mask != nullptr && (!CNT || a.countCulled != 0);
+        // coverage bits of the tile's patches (bit y * ts/8 + x)
+        unsigned long long pbits = ~0ull;
         if constexpr (TOP) {
             // copy the mesh's top levels once per change of mesh (made visible
             // by the barrier below)
@@ -1157,7 +1246,12 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 for (int k = tid; k < nTop * (kRecBytes<W> / 16); k += kWgSize) sTop[k] = src[k];
             }
         }
-        if constexpr (!REC && !CNT) {
+        if (useMask) {
+            pbits = tile_patch_bits(J, a.mask, tx, ty, ts);
+            skip = pbits == 0ull;
+            // (makes the staged top levels visible)
+            if constexpr (TOP) __syncthreads();
+        } else if constexpr (!REC && !CNT) {
             sCull[tid] = cull_rect_of(J, M, tid);
             __syncthreads();
             const int ia = J.i0 + tx * ts, ja = J.j0 + ty * ts;
@@ -1197,7 +1291,9 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                         const int w2 = q >> 6, l2 = q & 63;
                         ri = J.i0 + tx * ts + (pIdx % perSide) * 16 + (w2 & 1) * 8 + (l2 & 7);
                         rj = J.j0 + ty * ts + (pIdx / perSide) * 16 + (w2 >> 1) * 8 + (l2 >> 3);
-                        if (ri < J.nx && rj < J.ny) return true;
+                        // (rays of patches the coverage mask rules out are not traced)
+                        const int pb = ((pIdx / perSide) * 2 + (w2 >> 1)) * (ts >> 3) + (pIdx % perSide) * 2 + (w2 & 1);
+                        if (ri < J.nx && rj < J.ny && ((pbits >> pb) & 1ull)) return true;
                         sRes[2 * r] = ST(0);
                         sRes[2 * r + 1] = ST(0);
                     }
@@ -1293,7 +1389,9 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 const int i = J.i0 + tx * ts + px + (quad & 1) * 8 + (lane & 7);
                 const int j = J.j0 + ty * ts + py + (quad >> 1) * 8 + (lane >> 3);
                 bool reach = true;
-                if constexpr (!REC && !CNT) {
+                if (useMask) {
+                    reach = ((pbits >> (((py >> 3) + (quad >> 1)) * (ts >> 3) + (px >> 3) + (quad & 1))) & 1ull) != 0;
+                } else if constexpr (!REC && !CNT) {
                     const int pi = J.i0 + tx * ts + px + (quad & 1) * 8, pj = J.j0 + ty * ts + py + (quad >> 1) * 8;
                     reach = rays_reach_mesh(J, sCull, pi, min(pi + 7, J.nx - 1), pj, min(pj + 7, J.ny - 1), lane);
                 }
@@ -1674,6 +1772,21 @@ hipError_t launch_zero(unsigned* p, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(zero_words, dim3(grid), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
+
+// Coverage masks of a batched run's jobs; maxChunks: the most LDS chunks
+// of kCovLdsWords mask words any job of the run has.
+template <typename R>
+hipError_t launch_coverage(const Job<R>* jobs, const Mesh<R>* meshes, int nJobs, int maxChunks,
+                           unsigned long long* mask, hipStream_t s) {
+    if (nJobs <= 0 || maxChunks <= 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>(maxChunks), static_cast<unsigned>(nJobs < 65535 ? nJobs : 65535));
+    hipLaunchKernelGGL((coverage_kernel<R>), grid, dim3(256), 0, s, jobs, meshes, nJobs, mask);
+    return hipGetLastError();
+}
+template hipError_t launch_coverage<float>(const Job<float>*, const Mesh<float>*, int, int, unsigned long long*,
+                                           hipStream_t);
+template hipError_t launch_coverage<double>(const Job<double>*, const Mesh<double>*, int, int, unsigned long long*,
+                                            hipStream_t);
 
 template <int SD, int W> const void* mixed_fn(bool top) {
     return top ? reinterpret_cast<const void*>(&trace_kernel<float, 1, SD, false, false, true, false, true, true, W>)

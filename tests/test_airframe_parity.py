@@ -16,8 +16,9 @@ bounces) against references independent of libsbr's own kernels.
     oracle's sums (1e-9 relative).
 
 These cover the four-wide collapse of the 30-level tree, the leaf encoding
-at 3.2M triangles and the tile cull with checks that do not go through
-another libsbr kernel.
+at 3.2M triangles and the coverage masks of batched runs with checks that
+do not go through another libsbr kernel. Counts of the traced rays against
+counts over all rays show that the masks lose no bounce.
 """
 import numpy as np
 import pytest
@@ -127,3 +128,35 @@ def test_airframe_sigma_culled_vs_fp64_oracle(air, k):
             finally:
                 pl.close()
             np.testing.assert_allclose(z, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("k", [0, 1, 2], ids=FREQS)
+def test_airframe_coverage_masks_keep_every_hitting_ray(air, k):
+    """Batched runs skip the 8x8 ray patches that the jobs' coverage masks
+    (a 65,536-box cut of the airframe's tree) rule out: counting the rays
+    such a run traces must find every bounce that the count over all rays
+    of the culled rectangles finds, with the same sums."""
+    kw = air["wl"]["sweeps"][k][1]
+    angles = np.sort(air["picks"][k]).astype(np.int32)
+    p = S.SweepParams.make(**kw)
+    ctx, mesh = air["ctx"], air["mesh"]
+    for prec, name in ((S.SBR_FP64, "fp64"), (S.SBR_MIXED, "mixed")):
+        js = S.sweep_jobs(p, mesh, prec)
+        sub = (S.Job * len(angles))(*[js[int(a)] for a in angles])
+        pl = ctx.plan(sub, prec, S.SBR_BATCHED)
+        try:
+            _step(f"{FREQS[k]}: libsbr {name}, count over all rays")
+            full = pl.count()
+            z_all, _, _ = pl.results()
+            for ts in (16, 64):
+                _step(f"{FREQS[k]}: libsbr {name}, count over the traced rays, tile {ts}")
+                pl.tune(tile_rays=ts)
+                tr = pl.count(traced=True)
+                z_tr, _, _ = pl.results()
+                print(f"    traced rays {tr['rays']} of {full['rays']} ({tr['rays'] / full['rays']:.3f}), "
+                      f"node visits {tr['node_visits'] / full['node_visits']:.3f}", flush=True)
+                assert tr["ray_bounces"] == full["ray_bounces"] > 0
+                assert tr["rays"] < full["rays"]
+                assert np.array_equal(z_tr, z_all)
+        finally:
+            pl.close()

@@ -382,11 +382,12 @@ def test_graph_replay_after_mesh_table_grows():
 
 @pytest.mark.parametrize("offset", [0.0, 4e3, 4e4])
 def test_tile_cull_far_offset_mesh(ctx, offset):
-    """The per-tile emptiness test skips tiles whose rays cannot reach the
-    mesh. Its FP32 rounding grows with the mesh's distance from the origin
-    in grid sizes; for a body far down the ray path the sums with the test
-    (a timed run) must equal those of a counting run, which traces every
-    ray."""
+    """Batched runs skip tiles and ray patches whose rays cannot reach the
+    mesh (coverage masks). The FP32 rounding of the box projections grows
+    with the mesh's distance from the origin in grid sizes; for a body far
+    down the ray path the sums with the masks (a timed run, and a count of
+    the traced rays) must equal those of a counting run, which traces every
+    ray, and no bounce may be lost."""
     v, f = M.icosphere(3)
     grid = 2.5
     th, ph = np.radians(90.0), np.radians(20.0)
@@ -405,12 +406,47 @@ def test_tile_cull_far_offset_mesh(ctx, offset):
         pl.run()
         pl.wait()
         z_cull, _, _ = pl.results()
-        pl.count()
+        full = pl.count()
         z_all, _, _ = pl.results()
+        tr = pl.count(traced=True)
+        z_tr, _, _ = pl.results()
         pl.close()
         assert np.all(np.abs(z_all) > 0)
         assert np.array_equal(z_cull, z_all)
+        assert np.array_equal(z_tr, z_all)
+        assert tr["ray_bounces"] == full["ray_bounces"]
     ctx.destroy_mesh(mesh)
+
+
+@pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_MIXED, S.SBR_FP64])
+def test_coverage_masks_keep_every_hitting_ray(ctx, tri_duct, prec):
+    """Batched runs skip the 8x8 ray patches that their jobs' coverage masks
+    rule out. Counting the rays such a run traces must find every bounce
+    that the count over all rays finds, over fewer rays, with the same sums,
+    at every tile side; a timed run gives the same sums too."""
+    v, f, mesh, s32, s64 = tri_duct
+    p = S.SweepParams.make(**dict(C2S, theta_samples=4, phi_samples=4))
+    pl = ctx.plan(S.sweep_jobs(p, mesh, prec), prec, S.SBR_BATCHED)
+    full = pl.count()
+    z_all, _, _ = pl.results()
+    assert full["ray_bounces"] > 0
+    for ts in (16, 32, 64):
+        _step(f"precision {prec}, tile {ts}")
+        pl.tune(tile_rays=ts)
+        tr = pl.count(traced=True)
+        z_tr, _, _ = pl.results()
+        assert tr["ray_bounces"] == full["ray_bounces"]
+        assert 0 < tr["rays"] < full["rays"]
+        assert np.array_equal(z_tr, z_all)
+        pl.run()
+        pl.wait()
+        z_run, _, _ = pl.results()
+        assert np.array_equal(z_run, z_all)
+    pl.close()
+    per = ctx.plan(S.sweep_jobs(p, mesh, prec), prec, S.SBR_PER_ANGLE)
+    with pytest.raises(S.SbrError):
+        per.count(traced=True)
+    per.close()
 
 
 @pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_MIXED, S.SBR_FP64])

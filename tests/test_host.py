@@ -21,7 +21,7 @@ def test_library_exports_every_header_symbol():
     L = S.lib()
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
-    assert L.sbr_abi_version() == 4
+    assert L.sbr_abi_version() == 5
 
 
 @pytest.mark.parametrize("name", ["ref_box", "ref_sphere", "ico3", "trihedral8", "random_soup"])
