@@ -245,7 +245,7 @@ def pmc_measure(workload, tuning):
 
 def apply_tuning(plan, t):
     plan.tune(t["wg_per_cu"], t["lds_stack"], t["tile_rays"], t["use_graph"], t.get("ray_pool", 0),
-              t.get("traversal", 0), t.get("top_levels", 0), t.get("node_width", 0))
+              t.get("traversal", 0), t.get("top_levels", 0), t.get("node_width", 0), t.get("cull", 0))
 
 
 def build_workload(S, wl, ctx):

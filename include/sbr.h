@@ -99,6 +99,11 @@ typedef struct {
     int node_width;  /* SBR_MIXED, one ray per lane: children per record of the FP32 box
                         tree the mixed query walks: 2 (the binary tree), 4 (default) or 8;
                         a width is built (allocating) the first time a plan uses it */
+    int cull;        /* batched plans: how tiles and 8 x 8 ray patches that no ray of theirs
+                        can reach the mesh through are found and skipped: 1 = the jobs'
+                        coverage masks, built by each run ahead of the trace launch from up
+                        to 65,536 boxes per mesh (default); 2 = each tile tests 256 boxes
+                        (per-angle plans always do) */
 } sbr_tuning;
 
 /* Mesh / BVH statistics. */
@@ -215,9 +220,9 @@ int sbr_plan_results(sbr_plan* plan, double* reim, double* sigma, double* dbsm);
  * of its grid (src/cuda/rtKernels.cu:105-158). */
 int sbr_plan_count(sbr_plan* plan, sbr_counters* out);
 /* The same counters over the rays a timed run of the plan traces: tiles and
- * 8 x 8 ray patches that the coverage masks rule out (no ray of theirs can
- * reach a triangle) are skipped as in timed runs. Batched plans only. No
- * counterpart in the reference; it gives the work one timed launch does. */
+ * 8 x 8 ray patches that its cull method rules out (no ray of theirs can
+ * reach a triangle) are skipped as in timed runs. No counterpart in the
+ * reference; it gives the work one timed launch does. */
 int sbr_plan_count_traced(sbr_plan* plan, sbr_counters* out);
 int sbr_plan_destroy(sbr_plan* plan);
 
@@ -228,7 +233,8 @@ int sbr_plan_destroy(sbr_plan* plan);
  * x tile side {16, 32} x hipGraph replay {0, 1} (per-angle); then the other
  * traversal loop shape at the best of those; then, for SBR_MIXED, the top 3
  * or 4 tree levels staged in LDS at that shape and with an 8-entry LDS
- * stack, then node widths 2 and 8 at the best so far. Keeps the fastest and
+ * stack, then node widths 2 and 8 at the best so far; then, for batched
+ * plans, the other cull method. Keeps the fastest and
  * applies it to the plan. sample_jobs > 0
  * tunes on an evenly strided subset of that many jobs (for large sweeps);
  * repeats >= 1 timed runs per candidate (the minimum wall time counts).

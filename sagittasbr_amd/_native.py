@@ -96,7 +96,8 @@ ABI_VERSION = 5  # SBR_ABI_VERSION of the structures below
 
 class Tuning(C.Structure):
     _fields_ = [("wg_per_cu", C.c_int), ("lds_stack", C.c_int), ("tile_rays", C.c_int), ("use_graph", C.c_int),
-                ("ray_pool", C.c_int), ("traversal", C.c_int), ("top_levels", C.c_int), ("node_width", C.c_int)]
+                ("ray_pool", C.c_int), ("traversal", C.c_int), ("top_levels", C.c_int), ("node_width", C.c_int),
+                ("cull", C.c_int)]
 
 
 class MeshInfo(C.Structure):
@@ -296,8 +297,8 @@ class Plan:
         _check(lib().sbr_plan_create(ctx.h, C.cast(jobs, _P), self.n, precision, method, C.byref(self.h)), ctx.h)
 
     def tune(self, wg_per_cu=0, lds_stack=0, tile_rays=0, use_graph=0, ray_pool=0, traversal=0, top_levels=0,
-             node_width=0):
-        t = Tuning(wg_per_cu, lds_stack, tile_rays, use_graph, ray_pool, traversal, top_levels, node_width)
+             node_width=0, cull=0):
+        t = Tuning(wg_per_cu, lds_stack, tile_rays, use_graph, ray_pool, traversal, top_levels, node_width, cull)
         _check(lib().sbr_plan_set_tuning(self.h, C.byref(t)), self.ctx.h)
 
     def autotune(self, sample_jobs=0, repeats=1):
@@ -308,7 +309,7 @@ class Plan:
                self.ctx.h)
         return {"wg_per_cu": t.wg_per_cu, "lds_stack": t.lds_stack, "tile_rays": t.tile_rays,
                 "use_graph": t.use_graph, "ray_pool": t.ray_pool, "traversal": t.traversal,
-                "top_levels": t.top_levels, "node_width": t.node_width, "best_ms": ms.value,
+                "top_levels": t.top_levels, "node_width": t.node_width, "cull": t.cull, "best_ms": ms.value,
                 "candidates": n.value}
 
     def run(self):
@@ -331,8 +332,7 @@ class Plan:
 
     def count(self, traced=False):
         """Work counters of one run: over every ray of each job's culled
-        rectangle, or (traced=True, batched plans) over the rays a timed run
-        traces."""
+        rectangle, or (traced=True) over the rays a timed run traces."""
         c = Counters()
         fn = lib().sbr_plan_count_traced if traced else lib().sbr_plan_count
         _check(fn(self.h, C.byref(c)), self.ctx.h)

@@ -148,7 +148,7 @@ struct sbr_plan {
     DevBuf dChunkJob;  // job of the first tile of each kChunkTiles chunk
     int64_t nChunks = 0;
     int maxDepth = 0;
-    sbr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0};
+    sbr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0, 0};
     int tileSide = 32;
     DevBuf dJobs, dPartials, dJobOut, dCounter, dCounters;
     // mixed mode
@@ -727,7 +727,8 @@ DevPlan dev_of(sbr_plan* pl) {
     d.chunkJob = static_cast<const int*>(pl->dChunkJob.p);
     d.jobs64 = pl->dJobs64.p;
     d.recFlag = static_cast<int8_t*>(pl->dRecFlag.p);
-    d.mask = pl->method == SBR_BATCHED ? static_cast<unsigned long long*>(pl->dMask.p) : nullptr;
+    d.mask = pl->method == SBR_BATCHED && pl->tune.cull != 2 ? static_cast<unsigned long long*>(pl->dMask.p)
+                                                             : nullptr;
     return d;
 }
 
@@ -804,10 +805,12 @@ template <typename R> int plan_enqueue(sbr_plan* pl, const DevPlan& d, bool reco
         a.recFlag = d.recFlag;
     }
     // batched runs cull against the jobs' coverage masks, built first
+    // (unless tuned to the per-tile box tests); a counting run culls only
+    // when it counts the traced rays
     const bool masked = pl->method == SBR_BATCHED && !record && d.mask && pl->totalTiles > 0 &&
                         (!count || pl->countTraced);
     a.mask = masked ? d.mask : nullptr;
-    a.countCulled = count && masked ? 1 : 0;
+    a.countCulled = count && pl->countTraced ? 1 : 0;
     hipStream_t s = ctx->stream;
     HIP_TRY(&ctx->err, launch_zero(reinterpret_cast<unsigned*>(d.tileCounter), 2, s));
     HIP_TRY(&ctx->err, launch_zero(reinterpret_cast<unsigned*>(d.counters), 32, s));
@@ -1353,6 +1356,7 @@ int sbr_plan_set_tuning(sbr_plan* pl, const sbr_tuning* t) {
         return fail(&pl->ctx->err, "node_width must be 0, 2, 4 or 8");
     if (t->tile_rays != 0 && t->tile_rays != 16 && t->tile_rays != 32 && t->tile_rays != 64)
         return fail(&pl->ctx->err, "tile_rays must be 16, 32 or 64");
+    if (t->cull < 0 || t->cull > 2) return fail(&pl->ctx->err, "cull must be 0, 1 or 2");
     drop_graph(pl);
     const sbr_tuning old = pl->tune;
     const int oldSide = pl->tileSide;
@@ -1417,8 +1421,6 @@ static int plan_count(sbr_plan* pl, sbr_counters* out, bool traced) {
     if (!pl || !out) return fail(nullptr, "null argument");
     if (!pl->ctx) return fail(nullptr, "the plan's context was destroyed");
     sbr_ctx* ctx = pl->ctx;
-    if (traced && pl->method != SBR_BATCHED)
-        return fail(&ctx->err, "counting the traced rays needs a batched plan (coverage masks)");
     HIP_TRY(&ctx->err, hipSetDevice(ctx->device));
     pl->countTraced = traced;
     const int rc = dispatch_launch(pl, false, true);
@@ -1464,10 +1466,10 @@ int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* be
         for (int sd : {8, 16, 32}) {
             if (pl->method == SBR_BATCHED) {
                 for (int ts : {16, 32, 64})
-                    for (int wg : {0, 2}) cands.push_back(sbr_tuning{wg, sd, ts, 0, pool, 0, 0, 0});
+                    for (int wg : {0, 2}) cands.push_back(sbr_tuning{wg, sd, ts, 0, pool, 0, 0, 0, 0});
             } else {
                 for (int ts : {16, 32})
-                    for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool, 0, 0, 0});
+                    for (int g : {0, 1}) cands.push_back(sbr_tuning{0, sd, ts, g, pool, 0, 0, 0, 0});
             }
         }
     double bestT = 1e300;
@@ -1523,6 +1525,12 @@ int sbr_plan_autotune(sbr_plan* pl, int sample_jobs, int repeats, sbr_tuning* be
             c.node_width = w;
             if (!rc) rc = trial(c);
         }
+    }
+    // stage 5 (batched): the per-tile box tests instead of the coverage masks
+    if (!rc && pl->method == SBR_BATCHED) {
+        sbr_tuning c = bestC;
+        c.cull = 2;
+        rc = trial(c);
     }
     if (owned) sbr_plan_destroy(owned);
     if (rc) return rc;

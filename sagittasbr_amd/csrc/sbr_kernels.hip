@@ -1173,7 +1173,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     // next 8x8 ray patch of the current tile (timed runs, one ray per lane)
     __shared__ int sPatch;
     // projected cull boxes of the current tile's job (one per thread)
-    __shared__ CullRect sCull[(REC || CNT) ? 1 : kCullBoxes];
+    __shared__ CullRect sCull[REC ? 1 : kCullBoxes];
     // the top levels of the current mesh's W-wide tree (TOP)
     __shared__ vf4 sTop[TOP ? kTopBytes / 16 : 1];
     int stagedMesh = -1, nTop = 0;
@@ -1227,11 +1227,12 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
         // tiles and 8x8 ray patches none of whose rays can reach the mesh
         // are not traced (not in record runs, which account for every ray,
         // nor in counting runs unless they count what a timed run traces).
-        // Batched runs read the job's coverage mask; the others test the
+        // Runs with coverage masks read the job's mask; the others test the
         // kCullBoxes rectangles, into which every wave may write once it has
         // passed the previous tile's last barrier.
         bool skip = false;
-        const bool useMask = !REC && a.mask != nullptr && (!CNT || a.countCulled != 0);
+        const bool culled = !REC && (!CNT || a.countCulled != 0);
+        const bool useMask = culled && a.mask != nullptr;
         // coverage bits of the tile's patches (bit y * ts/8 + x)
         unsigned long long pbits = ~0ull;
         if constexpr (TOP) {
@@ -1251,11 +1252,14 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
             skip = pbits == 0ull;
             // (makes the staged top levels visible)
             if constexpr (TOP) __syncthreads();
-        } else if constexpr (!REC && !CNT) {
-            sCull[tid] = cull_rect_of(J, M, tid);
-            __syncthreads();
-            const int ia = J.i0 + tx * ts, ja = J.j0 + ty * ts;
-            skip = !rays_reach_mesh(J, sCull, ia, min(ia + ts - 1, J.nx - 1), ja, min(ja + ts - 1, J.ny - 1), lane);
+        } else if constexpr (!REC) {
+            if (culled) {
+                sCull[tid] = cull_rect_of(J, M, tid);
+                __syncthreads();
+                const int ia = J.i0 + tx * ts, ja = J.j0 + ty * ts;
+                skip = !rays_reach_mesh(J, sCull, ia, min(ia + ts - 1, J.nx - 1), ja, min(ja + ts - 1, J.ny - 1),
+                                        lane);
+            }
         }
 
         using AccT = Acc<R, MODE>;
@@ -1391,9 +1395,12 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 bool reach = true;
                 if (useMask) {
                     reach = ((pbits >> (((py >> 3) + (quad >> 1)) * (ts >> 3) + (px >> 3) + (quad & 1))) & 1ull) != 0;
-                } else if constexpr (!REC && !CNT) {
-                    const int pi = J.i0 + tx * ts + px + (quad & 1) * 8, pj = J.j0 + ty * ts + py + (quad >> 1) * 8;
-                    reach = rays_reach_mesh(J, sCull, pi, min(pi + 7, J.nx - 1), pj, min(pj + 7, J.ny - 1), lane);
+                } else if constexpr (!REC) {
+                    if (culled) {
+                        const int pi = J.i0 + tx * ts + px + (quad & 1) * 8,
+                                  pj = J.j0 + ty * ts + py + (quad >> 1) * 8;
+                        reach = rays_reach_mesh(J, sCull, pi, min(pi + 7, J.nx - 1), pj, min(pj + 7, J.ny - 1), lane);
+                    }
                 }
                 do {
                     if (!reach || i >= J.nx || j >= J.ny) break;

@@ -421,9 +421,10 @@ def test_tile_cull_far_offset_mesh(ctx, offset):
 @pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_MIXED, S.SBR_FP64])
 def test_coverage_masks_keep_every_hitting_ray(ctx, tri_duct, prec):
     """Batched runs skip the 8x8 ray patches that their jobs' coverage masks
-    rule out. Counting the rays such a run traces must find every bounce
-    that the count over all rays finds, over fewer rays, with the same sums,
-    at every tile side; a timed run gives the same sums too."""
+    rule out (or, tuned so, that the per-tile tests of 256 boxes rule out,
+    as per-angle runs do). Counting the rays such a run traces must find
+    every bounce that the count over all rays finds, over fewer rays, with
+    the same sums, at every tile side; a timed run gives the same sums too."""
     v, f, mesh, s32, s64 = tri_duct
     p = S.SweepParams.make(**dict(C2S, theta_samples=4, phi_samples=4))
     pl = ctx.plan(S.sweep_jobs(p, mesh, prec), prec, S.SBR_BATCHED)
@@ -442,11 +443,27 @@ def test_coverage_masks_keep_every_hitting_ray(ctx, tri_duct, prec):
         pl.wait()
         z_run, _, _ = pl.results()
         assert np.array_equal(z_run, z_all)
+        # the per-tile box tests instead of the masks: the same bounces and sums
+        # (either may trace a few more rays than the other: the masks' ray
+        # ranges carry one ray of margin on each side)
+        pl.tune(tile_rays=ts, cull=2)
+        tb = pl.count(traced=True)
+        z_tb, _, _ = pl.results()
+        assert tb["ray_bounces"] == full["ray_bounces"]
+        assert 0 < tb["rays"] < full["rays"]
+        assert np.array_equal(z_tb, z_all)
+        pl.run()
+        pl.wait()
+        z_run, _, _ = pl.results()
+        assert np.array_equal(z_run, z_all)
     pl.close()
     per = ctx.plan(S.sweep_jobs(p, mesh, prec), prec, S.SBR_PER_ANGLE)
-    with pytest.raises(S.SbrError):
-        per.count(traced=True)
+    tp = per.count(traced=True)
+    z_per, _, _ = per.results()
     per.close()
+    assert tp["ray_bounces"] == full["ray_bounces"]
+    assert tp["rays"] < full["rays"]
+    assert np.array_equal(z_per, z_all)
 
 
 @pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_MIXED, S.SBR_FP64])
@@ -489,6 +506,15 @@ def test_tuning_variants(ctx, tri_duct, prec):
             pl.wait()
             z, _, _ = pl.results()
             assert np.array_equal(z, z0), (top, trav)
+    # both cull methods
+    for cull in (2, 1):
+        for ts in (16, 64):
+            _step(f"precision {prec} batched cull={cull} tile={ts}")
+            pl.tune(cull=cull, tile_rays=ts, top_levels=4 if cull == 2 else 0)
+            pl.run()
+            pl.wait()
+            z, _, _ = pl.results()
+            assert np.array_equal(z, z0), (cull, ts)
     # the mixed mode's binary and eight-wide trees (ignored elsewhere); the
     # eight-wide tree's top 3 levels do not all fit the LDS copy
     for width in (2, 8, 4):
@@ -534,12 +560,13 @@ def test_autotune(ctx, tri_duct):
         print("   ", best, flush=True)
         # launch shapes, the other loop shape, then (mixed, no ray pool) the
         # top tree levels in LDS at 3 or 4 levels, also with an 8-entry stack,
-        # then node widths 2 and 8
-        base = (36 if method == S.SBR_BATCHED else 24) + 1
+        # then node widths 2 and 8, then (batched) the other cull method
+        base = (36 + 1 + 1 if method == S.SBR_BATCHED else 24 + 1)
         assert best["candidates"] in {base + a + b for a in (0, 2, 4) for b in (0, 2)}
         assert best["tile_rays"] in (16, 32, 64) and best["lds_stack"] in (8, 16, 32) and best["best_ms"] > 0
         assert best["ray_pool"] in (1, 2) and best["traversal"] in (0, 1) and best["top_levels"] in (0, 3, 4)
         assert best["node_width"] in (0, 2, 8)
+        assert best["cull"] in ((0, 2) if method == S.SBR_BATCHED else (0,))
         pl.run()
         pl.wait()
         z, _, _ = pl.results()
