@@ -1248,7 +1248,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
             }
         }
         if (useMask) {
-            pbits = tile_patch_bits(J, a.mask, tx, ty, ts);
+            // (the same for every lane: kept in scalar registers)
+            pbits = static_cast<unsigned long long>(uniform64(static_cast<int64_t>(tile_patch_bits(J, a.mask, tx, ty, ts))));
             skip = pbits == 0ull;
             // (makes the staged top levels visible)
             if constexpr (TOP) __syncthreads();
