@@ -127,6 +127,20 @@ template <typename T> using gptr = const __attribute__((address_space(1))) T*;
 template <typename T> __device__ __forceinline__ gptr<T> as_global(const void* p) {
     return (gptr<T>)(p);
 }
+// The same value in every lane, read from the first active lane (held in
+// scalar registers).
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+    unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<uint64_t>(v)));
+    unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<uint64_t>(v) >> 32));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+// A pointer every lane of the wavefront holds (a mesh's arrays: a wave
+// traces rays of one job), kept in scalar registers; the compiler cannot
+// tell that a pointer loaded from the mesh table is the same in every lane.
+template <typename T> __device__ __forceinline__ T* uniform_ptr(T* p) {
+    return reinterpret_cast<T*>(uniform64(reinterpret_cast<int64_t>(p)));
+}
+
 typedef float vf4 __attribute__((ext_vector_type(4)));
 typedef double vd2 __attribute__((ext_vector_type(2)));
 typedef int vi4 __attribute__((ext_vector_type(4)));
@@ -369,8 +383,9 @@ template <typename R, int SD> struct LaneStack {
 template <typename R, int SD, bool CNT, bool WW, bool PF = true>
 __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V3<R> inv, R& tHit, int* sRef, R* sE,
                                            typename Ov<R>::T* ov, int tid, Work& w) {
-    const R* nodes = static_cast<const R*>(m.nodes);
-    const R* tris = static_cast<const R*>(m.tris);
+    // (the mesh's arrays in scalar registers)
+    const R* nodes = uniform_ptr(static_cast<const R*>(m.nodes));
+    const R* tris = uniform_ptr(static_cast<const R*>(m.tris));
     R closest = R(1e20);
     int hit = -1;
     if (CNT) w.visits += 1;
@@ -524,7 +539,16 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     float lim = 3.0e38f;
     auto update_lim = [&]() { lim = float(closest - shift) * (1.0f + 4.0f * kEps32) + 2.0f * tErr; };
     LaneStack<float, SD> st{sRef, sE, ov, tid, 0};
-    const void* nodes = m.nodesW[width_slot(W)];
+    // (the mesh's arrays in scalar registers)
+    const void* nodes = uniform_ptr(m.nodesW[width_slot(W)]);
+    const void* tris = uniform_ptr(m.tris);
+    const void* tris64 = uniform_ptr(m.tris64);
+    const bool vertsExact = __builtin_amdgcn_readfirstlane(m.vertsExact) != 0;
+    // the key of a child not entered (3e38, above every finite bound), made
+    // in a scalar register: as a literal the compiler holds it in a vector
+    // register through the traversal
+    float noKey;
+    asm("s_mov_b32 %0, 0x7f61b1e6" : "=s"(noKey));
     int cur = m.rootRefW[width_slot(W)];
     // wide tree: the hit children are visited nearest first (the outcome
     // does not depend on the order)
@@ -542,7 +566,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
             const bool h = slab_grown(n.f[q], n.f[W + q], n.f[2 * W + q], n.f[3 * W + q], n.f[4 * W + q],
                                       n.f[5 * W + q], pLo, pHi, inv, ent) &&
                            ent <= lim;
-            e[q] = h ? ent : 3.0e38f;
+            e[q] = h ? ent : noKey;
             r[q] = n.ref(q);
         }
         sort_net<W>(e, r);
@@ -572,13 +596,13 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
                 }
             }
         };
-        if (m.vertsExact) {
+        if (vertsExact) {
             // FP32 records widened: the next triangle's record is loaded
             // while this one is tested
             // (the nine vertex floats of the 48-byte record: two 16-byte loads
             // and one 4-byte load)
-            const gptr<vf4> p = as_global<vf4>(m.tris) + 3 * first;
-            const gptr<float> pf = as_global<float>(m.tris) + 12 * first;
+            const gptr<vf4> p = as_global<vf4>(tris) + 3 * first;
+            const gptr<float> pf = as_global<float>(tris) + 12 * first;
             vf4 ra = p[0], rb = p[1];
             float rc = pf[8];
             for (int k = 0; k < cnt; ++k) {
@@ -600,7 +624,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
             for (int k = 0; k < cnt; ++k) {
                 V3<double> a, b, c;
                 int f;
-                load_tri_raw64(m.tris64, first + k, a, b, c, f);
+                load_tri_raw64(tris64, first + k, a, b, c, f);
                 test(a, b, c, k);
             }
         }
@@ -716,11 +740,6 @@ template <typename A> __device__ __forceinline__ A wg_sum(A a, A* red, int tid) 
 // Values every lane has just read from LDS after a barrier, handed to the
 // compiler as wave-uniform scalars so that the loop exit and the branches
 // around barriers that depend on them are uniform branches.
-__device__ __forceinline__ int64_t uniform64(int64_t v) {
-    unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<uint64_t>(v)));
-    unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<uint64_t>(v) >> 32));
-    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
-}
 
 
 // ------------------------------------------------------------ ray paths
@@ -1242,7 +1261,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                 stagedMesh = J.mesh;
                 // (a prefix of the breadth-first top records when the levels
                 // do not fit)
-                nTop = min(M.topRecsW[width_slot(W)][a.topLevels], kTopBytes / kRecBytes<W>);
+                nTop = __builtin_amdgcn_readfirstlane(
+                    min(M.topRecsW[width_slot(W)][a.topLevels], kTopBytes / kRecBytes<W>));
                 const gptr<vf4> src = as_global<vf4>(M.nodesW[width_slot(W)]);
                 for (int k = tid; k < nTop * (kRecBytes<W> / 16); k += kWgSize) sTop[k] = src[k];
             }

@@ -247,6 +247,39 @@ def test_job_sums_independent_of_neighbours(ctx, tri_duct, prec):
     assert np.array_equal(z[2:], z0[::-1])
 
 
+@pytest.mark.parametrize("prec", [S.SBR_MIXED, S.SBR_FP32])
+def test_plan_beyond_one_grid_row_of_jobs(ctx, sphere, prec):
+    """A plan of more jobs than one launch grid dimension holds (65,535):
+    the coverage and job-sum launches stride over the jobs. Jobs past the
+    first 65,535 give the same sums as in a plan of their own, and the
+    masks lose no bounce there."""
+    v, f, mesh, s32, s64 = sphere
+    p = S.SweepParams.make(theta_start=0, theta_end=180, theta_samples=190, phi_start=0, phi_end=359,
+                           phi_samples=360, grid_size=2.5, n_x=16, n_y=16, max_bounces=2, freq=3e9)
+    jobs = S.sweep_jobs(p, mesh, prec)
+    n = len(jobs)
+    assert n > 65535
+    _step(f"precision {prec}: {n} jobs in one batched plan")
+    pl = ctx.plan(jobs, prec, S.SBR_BATCHED)
+    pl.run()
+    pl.wait()
+    z, _, _ = pl.results()
+    full = pl.count()
+    tr = pl.count(traced=True)
+    pl.close()
+    assert tr["ray_bounces"] == full["ray_bounces"] > 0
+    for lo, hi in ((65530, n), (0, 2000)):
+        _step(f"precision {prec}: jobs {lo}..{hi} on their own")
+        sub = (S.Job * (hi - lo))(*[jobs[k] for k in range(lo, hi)])
+        ps = ctx.plan(sub, prec, S.SBR_BATCHED)
+        ps.run()
+        ps.wait()
+        zs, _, _ = ps.results()
+        ps.close()
+        assert np.array_equal(zs, z[lo:hi])
+        assert np.all(np.abs(zs) > 0)
+
+
 def test_tri_duct_sigma_fp64_and_methods(ctx, tri_duct):
     v, f, mesh, s32, s64 = tri_duct
     kw = dict(C2S, theta_samples=4, phi_samples=4, theta_end=90, phi_end=90)
