@@ -206,11 +206,28 @@ void widen_boxes(GpuBvhW& g) {
 
 // The mixed mode's wide collapse of the mesh's FP32 tree at this width,
 // built (on the host, then uploaded) on first use.
+// Every node and triangle array of a built tree below 4 GiB (the kernels
+// address records with 32-bit byte offsets); an unbuilt tree passes.
+constexpr uint64_t kRecordSpan = uint64_t(1) << 32;
+template <typename R> bool records_fit(const PrecMesh<R>& pm) {
+    if (!pm.built) return true;
+    if (pm.gpu.nodes.size() >= kRecordSpan || pm.gpu.tris.size() >= kRecordSpan) return false;
+    if constexpr (std::is_same_v<R, float>) {
+        if (!pm.vertsExact && uint64_t(pm.bvh.perm.size()) * 80u >= kRecordSpan) return false;
+    }
+    return true;
+}
+
 int ensure_width(sbr_ctx* ctx, PrecMesh<float>& pm, int width) {
     const int slot = width_slot(width);
     if (pm.builtW[slot]) return 0;
     std::string err;
     if (!to_gpu_layout_wide(pm.bvh, width, pm.gpuW[slot], err)) return fail(&ctx->err, err);
+    // the kernels address wide records with 32-bit byte offsets
+    if (uint64_t(pm.gpuW[slot].nodes.size()) >= (uint64_t(1) << 32)) {
+        pm.gpuW[slot] = GpuBvhW{};
+        return fail(&ctx->err, "the " + std::to_string(width) + "-wide tree exceeds 4 GiB");
+    }
     if (!pm.vertsExact) widen_boxes(pm.gpuW[slot]);
     HIP_TRY(&ctx->err, pm.nodesW[slot].ensure(pm.gpuW[slot].nodes.size()));
     HIP_TRY(&ctx->err, hipMemcpy(pm.nodesW[slot].p, pm.gpuW[slot].nodes.data(), pm.gpuW[slot].nodes.size(),
@@ -609,6 +626,10 @@ template <typename R> int plan_host(sbr_plan* pl, bool record, bool noAlloc, Hos
             J.ex0 = float(kErrSafety * 5.9604645e-8 * 4.0 * (nrm(llc2) + nrm(D.uv) + nrm(D.vv)));
             jobs64[i] = D;
         }
+        // the kernels address node and triangle records with 32-bit byte offsets
+        if (!records_fit(pm) || (mixed && !records_fit(m.f64)))
+            return fail(&ctx->err, "mesh " + std::to_string(j.mesh) +
+                                       ": a node or triangle array of its trees exceeds the 4 GiB the kernels address");
         J.nx = j.n_x;
         J.ny = j.n_y;
         J.maxB = j.max_bounces;

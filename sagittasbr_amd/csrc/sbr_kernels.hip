@@ -127,6 +127,12 @@ template <typename T> using gptr = const __attribute__((address_space(1))) T*;
 template <typename T> __device__ __forceinline__ gptr<T> as_global(const void* p) {
     return (gptr<T>)(p);
 }
+// Byte offsets into a mesh's node and triangle records are 32-bit (the host
+// keeps every such array a plan reads below 4 GiB), so a load addresses the
+// array's scalar base plus one 32-bit vector register.
+template <typename T> __device__ __forceinline__ gptr<T> at_byte(const void* base, uint32_t off) {
+    return reinterpret_cast<gptr<T>>(as_global<char>(base) + off);
+}
 // The same value in every lane, read from the first active lane (held in
 // scalar registers).
 __device__ __forceinline__ int64_t uniform64(int64_t v) {
@@ -153,16 +159,17 @@ template <typename R> struct TriRec {
 // FP32 record: v0, v1, v2, face, 8 bytes of padding; the edges are formed
 // here exactly as rtKernels.cu:13-14 forms them.
 __device__ __forceinline__ TriRec<float> load_tri(const float* tris, int i) {
-    gptr<vf4> p = as_global<vf4>(tris) + 3 * i;
-    vf4 a = p[0], b = p[1], c = p[2];
+    const uint32_t o = static_cast<uint32_t>(i) * 48u;
+    const vf4 a = *at_byte<vf4>(tris, o), b = *at_byte<vf4>(tris, o + 16u), c = *at_byte<vf4>(tris, o + 32u);
     V3<float> v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
     return {v0, sub(v1, v0), sub(v2, v0), __float_as_int(c.y)};
 }
 // The same record's three vertices as stored, and the face id.
 __device__ __forceinline__ void load_tri_raw64(const void* tris64, int i, V3<double>& v0, V3<double>& v1,
                                                V3<double>& v2, int& face) {
-    gptr<vd2> p = as_global<vd2>(tris64) + 5 * i;
-    vd2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    const uint32_t o = static_cast<uint32_t>(i) * 80u;
+    const vd2 a = *at_byte<vd2>(tris64, o), b = *at_byte<vd2>(tris64, o + 16u), c = *at_byte<vd2>(tris64, o + 32u),
+              d = *at_byte<vd2>(tris64, o + 48u), e = *at_byte<vd2>(tris64, o + 64u);
     v0 = {a.x, a.y, b.x};
     v1 = {b.y, c.x, c.y};
     v2 = {d.x, d.y, e.x};
@@ -174,8 +181,8 @@ __device__ __forceinline__ void load_tri_raw64(const void* tris64, int i, V3<dou
 __device__ __forceinline__ void load_tri_mixed(const Mesh<float>& m, int i, V3<double>& v0, V3<double>& v1,
                                                V3<double>& v2, int& face) {
     if (m.vertsExact) {
-        const gptr<vf4> p = as_global<vf4>(m.tris) + 3 * i;
-        const vf4 a = p[0], b = p[1], c = p[2];
+        const uint32_t o = static_cast<uint32_t>(i) * 48u;
+        const vf4 a = *at_byte<vf4>(m.tris, o), b = *at_byte<vf4>(m.tris, o + 16u), c = *at_byte<vf4>(m.tris, o + 32u);
         v0 = {double(a.x), double(a.y), double(a.z)};
         v1 = {double(a.w), double(b.x), double(b.y)};
         v2 = {double(b.z), double(b.w), double(c.x)};
@@ -185,8 +192,9 @@ __device__ __forceinline__ void load_tri_mixed(const Mesh<float>& m, int i, V3<d
     }
 }
 __device__ __forceinline__ TriRec<double> load_tri(const double* tris, int i) {
-    gptr<vd2> p = as_global<vd2>(tris) + 5 * i;
-    vd2 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    const uint32_t o = static_cast<uint32_t>(i) * 80u;
+    const vd2 a = *at_byte<vd2>(tris, o), b = *at_byte<vd2>(tris, o + 16u), c = *at_byte<vd2>(tris, o + 32u),
+              d = *at_byte<vd2>(tris, o + 48u), e = *at_byte<vd2>(tris, o + 64u);
     int face = static_cast<int>(__double_as_longlong(e.y) & 0xffffffffll);
     return {{a.x, a.y, b.x}, {b.y, c.x, c.y}, {d.x, d.y, e.x}, face};
 }
@@ -196,21 +204,21 @@ template <typename R> struct NodeRec {
     int l, r;
 };
 __device__ __forceinline__ NodeRec<float> load_node(const float* nodes, int i) {
-    gptr<vf4> p = as_global<vf4>(nodes) + 4 * i;
-    vf4 a = p[0], b = p[1], c = p[2];
-    vi4 d = as_global<vi4>(nodes)[4 * i + 3];
+    const uint32_t o = static_cast<uint32_t>(i) * 64u;
+    const vf4 a = *at_byte<vf4>(nodes, o), b = *at_byte<vf4>(nodes, o + 16u), c = *at_byte<vf4>(nodes, o + 32u);
+    const vi4 d = *at_byte<vi4>(nodes, o + 48u);
     return {{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w}, d.x, d.y};
 }
 __device__ __forceinline__ NodeRec<double> load_node(const double* nodes, int i) {
-    gptr<vd2> p = as_global<vd2>(nodes) + 8 * i;
+    const uint32_t o = static_cast<uint32_t>(i) * 128u;
     NodeRec<double> n;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        vd2 v = p[k];
+        const vd2 v = *at_byte<vd2>(nodes, o + 16u * k);
         n.b[2 * k] = v.x;
         n.b[2 * k + 1] = v.y;
     }
-    vi4 d = as_global<vi4>(nodes)[8 * i + 6];
+    const vi4 d = *at_byte<vi4>(nodes, o + 96u);
     n.l = d.x;
     n.r = d.y;
     return n;
@@ -226,7 +234,7 @@ template <int W> struct NodeW {
     __device__ __forceinline__ int ref(int q) const { return __float_as_int(f[6 * W + q]); }
 };
 template <int W> __device__ __forceinline__ NodeW<W> load_nodew(const void* nodes, int i) {
-    gptr<vf4> p = as_global<vf4>(nodes) + (kRecBytes<W> / 16) * i;
+    gptr<vf4> p = at_byte<vf4>(nodes, static_cast<uint32_t>(i) * static_cast<uint32_t>(kRecBytes<W>));
     NodeW<W> n;
 #pragma unroll
     for (int k = 0; k < NodeW<W>::kLoads; ++k) {
@@ -421,17 +429,17 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
             // one is tested (two 16-byte loads and one 4-byte load); the
             // edges are formed as load_tri forms them. PF false: the kernel
             // holds too many registers for the nine (the ray pool's)
-            const gptr<vf4> p = as_global<vf4>(tris) + 3 * first;
-            const gptr<float> pf = as_global<float>(tris) + 12 * first;
-            vf4 ra = p[0], rb = p[1];
-            float rc = pf[8];
+            const uint32_t ob = static_cast<uint32_t>(first) * 48u;
+            vf4 ra = *at_byte<vf4>(tris, ob), rb = *at_byte<vf4>(tris, ob + 16u);
+            float rc = *at_byte<float>(tris, ob + 32u);
             for (int k = 0; k < cnt; ++k) {
                 vf4 na = ra, nb = rb;
                 float nc = rc;
                 if (k + 1 < cnt) {
-                    na = p[3 * k + 3];
-                    nb = p[3 * k + 4];
-                    nc = pf[12 * k + 20];
+                    const uint32_t on = ob + 48u * static_cast<uint32_t>(k + 1);
+                    na = *at_byte<vf4>(tris, on);
+                    nb = *at_byte<vf4>(tris, on + 16u);
+                    nc = *at_byte<float>(tris, on + 32u);
                 }
                 const V3<float> v0{ra.x, ra.y, ra.z}, v1{ra.w, rb.x, rb.y}, v2{rb.z, rb.w, rc};
                 const TriRec<float> tr{v0, sub(v1, v0), sub(v2, v0), 0};
@@ -601,17 +609,17 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
             // while this one is tested
             // (the nine vertex floats of the 48-byte record: two 16-byte loads
             // and one 4-byte load)
-            const gptr<vf4> p = as_global<vf4>(tris) + 3 * first;
-            const gptr<float> pf = as_global<float>(tris) + 12 * first;
-            vf4 ra = p[0], rb = p[1];
-            float rc = pf[8];
+            const uint32_t ob = static_cast<uint32_t>(first) * 48u;
+            vf4 ra = *at_byte<vf4>(tris, ob), rb = *at_byte<vf4>(tris, ob + 16u);
+            float rc = *at_byte<float>(tris, ob + 32u);
             for (int k = 0; k < cnt; ++k) {
                 vf4 na = ra, nb = rb;
                 float nc = rc;
                 if (k + 1 < cnt) {
-                    na = p[3 * k + 3];
-                    nb = p[3 * k + 4];
-                    nc = pf[12 * k + 20];
+                    const uint32_t on = ob + 48u * static_cast<uint32_t>(k + 1);
+                    na = *at_byte<vf4>(tris, on);
+                    nb = *at_byte<vf4>(tris, on + 16u);
+                    nc = *at_byte<float>(tris, on + 32u);
                 }
                 test(V3<double>{double(ra.x), double(ra.y), double(ra.z)},
                      V3<double>{double(ra.w), double(rb.x), double(rb.y)},
