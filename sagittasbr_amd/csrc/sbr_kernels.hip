@@ -110,14 +110,20 @@ __device__ __forceinline__ bool slab(R lx, R ly, R lz, R hx, R hy, R hz, V3<R> o
 // rounded outwards and pLo = oLo * inv, pHi = oHi * inv formed once per
 // query by the caller; inv is finite). Its rounding, eps (|o| + |t|) in
 // position units, is covered by the position error bound dx.
+// The same test with the slab interval's far end capped at bound: true iff
+// entry <= far and entry <= bound (one compare; no value here is NaN).
 __device__ __forceinline__ bool slab_grown(float lx, float ly, float lz, float hx, float hy, float hz, V3<float> pLo,
-                                           V3<float> pHi, V3<float> inv, float& entry) {
+                                           V3<float> pHi, V3<float> inv, float bound, float& entry) {
     float t0x = fmaf(lx, inv.x, -pLo.x), t0y = fmaf(ly, inv.y, -pLo.y), t0z = fmaf(lz, inv.z, -pLo.z);
     float t1x = fmaf(hx, inv.x, -pHi.x), t1y = fmaf(hy, inv.y, -pHi.y), t1z = fmaf(hz, inv.z, -pHi.z);
     float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), bound));
     entry = fmaxf(tn, 0.001f);
     return entry <= tf;
+}
+__device__ __forceinline__ bool slab_grown(float lx, float ly, float lz, float hx, float hy, float hz, V3<float> pLo,
+                                           V3<float> pHi, V3<float> inv, float& entry) {
+    return slab_grown(lx, ly, lz, hx, hy, hz, pLo, pHi, inv, 3.0e38f, entry);
 }
 
 // BVH and triangle records are read through global-address-space pointers:
@@ -342,33 +348,36 @@ struct Work {
 constexpr int kNoRef = INT_MIN;
 
 // Deferred-left stack of one lane: entries [0, SD) in LDS at
-// [k * kWgSize + tid], deeper ones in the private overflow array.
+// [k * kWgSize + tid], deeper ones in the private overflow array. The lane
+// keeps at = sp * kWgSize + tid, the LDS index of its next entry: since
+// 0 <= tid < kWgSize, sp < SD exactly when at < SD * kWgSize, and sp is
+// at / kWgSize, so one register serves as stack pointer and LDS address.
 template <typename R, int SD> struct LaneStack {
+    static_assert(kWgSize == 256, "sp = at >> 8");
     int* sRef;
     R* sE;
     typename Ov<R>::T* ov;
-    int tid;
-    int sp;
+    int at;
     __device__ __forceinline__ void push(R e, int r) {
-        if (SD > 0 && sp < SD) {
-            sRef[sp * kWgSize + tid] = r;
-            sE[sp * kWgSize + tid] = e;
+        if (SD > 0 && at < SD * kWgSize) {
+            sRef[at] = r;
+            sE[at] = e;
         } else {
-            ov[sp - SD] = Ov<R>::pack(e, r);
+            ov[(at >> 8) - SD] = Ov<R>::pack(e, r);
         }
-        ++sp;
+        at += kWgSize;
     }
     // the next deferred child whose slab entry is <= lim, or kNoRef
     __device__ __forceinline__ int pop(R lim) {
-        while (sp > 0) {
-            --sp;
+        while (at >= kWgSize) {
+            at -= kWgSize;
             R ee;
             int rr;
-            if (SD > 0 && sp < SD) {
-                ee = sE[sp * kWgSize + tid];
-                rr = sRef[sp * kWgSize + tid];
+            if (SD > 0 && at < SD * kWgSize) {
+                ee = sE[at];
+                rr = sRef[at];
             } else {
-                Ov<R>::unpack(ov[sp - SD], ee, rr);
+                Ov<R>::unpack(ov[(at >> 8) - SD], ee, rr);
             }
             if (ee <= lim) return rr;
         }
@@ -399,7 +408,7 @@ __device__ __forceinline__ int closest_hit(const Mesh<R>& m, V3<R> o, V3<R> d, V
     if (CNT) w.visits += 1;
     R e;
     if (!slab(m.rootLo[0], m.rootLo[1], m.rootLo[2], m.rootHi[0], m.rootHi[1], m.rootHi[2], o, inv, e)) return -1;
-    LaneStack<R, SD> st{sRef, sE, ov, tid, 0};
+    LaneStack<R, SD> st{sRef, sE, ov, tid};
     int cur = m.rootRef;
     // one node step: descend into a hit child (true) or not (false)
     auto node_step = [&]() -> bool {
@@ -546,7 +555,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     // the tie margin, far below it)
     float lim = 3.0e38f;
     auto update_lim = [&]() { lim = float(closest - shift) * (1.0f + 4.0f * kEps32) + 2.0f * tErr; };
-    LaneStack<float, SD> st{sRef, sE, ov, tid, 0};
+    LaneStack<float, SD> st{sRef, sE, ov, tid};
     // (the mesh's arrays in scalar registers)
     const void* nodes = uniform_ptr(m.nodesW[width_slot(W)]);
     const void* tris = uniform_ptr(m.tris);
@@ -572,8 +581,7 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
             // an unused slot's box lies at +infinity: its entry is +infinity
             // (above the finite lim) or its slab interval is empty
             const bool h = slab_grown(n.f[q], n.f[W + q], n.f[2 * W + q], n.f[3 * W + q], n.f[4 * W + q],
-                                      n.f[5 * W + q], pLo, pHi, inv, ent) &&
-                           ent <= lim;
+                                      n.f[5 * W + q], pLo, pHi, inv, lim, ent);
             e[q] = h ? ent : noKey;
             r[q] = n.ref(q);
         }
@@ -1165,10 +1173,13 @@ constexpr int kPoolRays = 1024;  // rays per pool batch (4 passes of 16x16)
 // mode (1 = no constraint); overridable at build time. Measured on the
 // trihedral+duct sweep and a 10 GHz airframe sweep (LDS stack 8): the
 // unconstrained mixed kernel takes ~150 VGPRs (3 waves) and runs 12-16%
-// slower; held to 4 waves (128 VGPRs) it beats 5 waves by 4% on the
-// trihedral+duct sweep and ties on the airframe, and 6 waves (80 VGPRs,
-// spills in the traversal loop) loses 8%. FP64 gains ~10% at 4 waves, FP32
-// ~2-8% at 8. Deeper LDS stacks are bound by LDS instead.
+// slower; held to 4 waves (128 VGPRs) it beats 5 waves (9% on the airframe,
+// 26% on the trihedral+duct sweep, with its per-query spills) and 6 waves
+// (80 VGPRs, spills in the traversal loop: -8%). FP64 gains ~10% at 4 waves.
+// The FP32 kernels fit 6 waves (at most 80 VGPRs) without spills; asked for
+// 8 the allocator misses that target and the persistent kernel can land at
+// 85 VGPRs, 5 waves (batched FP32 on an airframe sample +8.7%). Deeper LDS
+// stacks are bound by LDS instead.
 #ifndef SBR_WPE_MIXED
 #define SBR_WPE_MIXED 4
 #endif
@@ -1176,7 +1187,7 @@ constexpr int kPoolRays = 1024;  // rays per pool batch (4 passes of 16x16)
 #define SBR_WPE_FP64 4
 #endif
 #ifndef SBR_WPE_FP32
-#define SBR_WPE_FP32 8
+#define SBR_WPE_FP32 6
 #endif
 template <typename R, int MODE> constexpr int kWavesPerEu =
     MODE == 1 ? SBR_WPE_MIXED : (std::is_same_v<R, double> ? SBR_WPE_FP64 : SBR_WPE_FP32);
@@ -1189,8 +1200,10 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     static_assert(W == kWide || (MODE == 1 && !POOL), "node widths other than the default: mixed mode, no pool");
     using ST = typename SlotOf<R, MODE>::T;
     using RS = typename std::conditional<MODE == 1, double, R>::type;  // ray state precision
-    __shared__ int sRef[SD * kWgSize];
-    __shared__ R sE[SD * kWgSize];
+    // (256-byte aligned: a stack entry's LDS address is then 4 x its index,
+    // the arrays' places going into the instructions' offset fields)
+    __shared__ __attribute__((aligned(256))) int sRef[SD * kWgSize];
+    __shared__ __attribute__((aligned(256))) R sE[SD * kWgSize];
     __shared__ Acc<R, MODE> red[kWgSize / 64];
     // wavefront sums of each 16x16 pass of a tile (at most 4 x 4 passes)
     __shared__ Acc<R, MODE> sPass[16 * (kWgSize / 64)];
