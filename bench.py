@@ -15,8 +15,11 @@ Prints one JSON line: angles/s of the whole sweep and ray-bounces/s, beside
 (on slice 0, 6,696 angles) the plain per-angle FP32 method, batched FP32 and
 FP64 on the same GPU, the dB error of the mixed and FP32 sweeps against FP64,
 the roofline of the trace kernel (algorithmic VALU operations of the
-reference traversal against the VALU peak; HBM traffic, VALU and L2 activity
-from rocprofv3 counter passes of the same kernel in child processes), and
+reference traversal over the rays the launch traces, counted by an FP64
+counting run with the timed run's culling, against the VALU peak; the same
+over every ray of the culled rectangles beside it; HBM traffic, VALU and L2
+activity from rocprofv3 counter passes of the same kernel in child
+processes), and
 the baseline leg: the CPU oracle on the host cores and the reference's own
 HIP kernels (built from its sources) on a seeded sample of the same sweep.
 
