@@ -247,6 +247,44 @@ def test_job_sums_independent_of_neighbours(ctx, tri_duct, prec):
     assert np.array_equal(z[2:], z0[::-1])
 
 
+@pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_FP64, S.SBR_MIXED])
+def test_non_square_ray_grid(ctx, tri_duct, prec):
+    """n_x != n_y (the reference's two config keys, include/app/config.hpp:7-27):
+    per ray against the oracle (the FP32 twin for FP32, the FP64 oracle
+    otherwise), and the batched sums with either cull method equal to the
+    per-angle sums, no bounce lost to the culling."""
+    v, f, mesh, s32, s64 = tri_duct
+    kw = dict(C2S, theta_samples=3, phi_samples=3, n_x=301, n_y=173)
+    p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    for a in (0, 4, 8):
+        _step(f"precision {prec} angle {a}: per ray, {p.n_x} x {p.n_y}")
+        g = ctx.trace_rays(mesh, p, a, prec)
+        o = s32.trace_angle(op, a, fused=True) if prec == S.SBR_FP32 else s64.trace_angle(op, a)
+        assert np.array_equal(g["count"], o["count"])
+        assert np.array_equal(g["ids"], o["ids"])
+        assert np.array_equal(g["L"], o["L"])
+        assert np.count_nonzero(g["count"]) > 1000
+    jobs = S.sweep_jobs(p, mesh, prec)
+    pa = ctx.plan(jobs, prec, S.SBR_PER_ANGLE)
+    pa.run()
+    pa.wait()
+    z0, _, _ = pa.results()
+    pa.close()
+    pl = ctx.plan(jobs, prec, S.SBR_BATCHED)
+    full = pl.count()
+    for cull in (1, 2):
+        for ts in (16, 64):
+            _step(f"precision {prec} batched cull={cull} tile={ts}")
+            pl.tune(cull=cull, tile_rays=ts)
+            pl.run()
+            pl.wait()
+            z, _, _ = pl.results()
+            assert np.array_equal(z, z0)
+            tr = pl.count(traced=True)
+            assert tr["ray_bounces"] == full["ray_bounces"] > 0
+    pl.close()
+
+
 @pytest.mark.parametrize("prec", [S.SBR_MIXED, S.SBR_FP32])
 def test_plan_beyond_one_grid_row_of_jobs(ctx, sphere, prec):
     """A plan of more jobs than one launch grid dimension holds (65,535):
