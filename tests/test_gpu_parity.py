@@ -247,6 +247,61 @@ def test_job_sums_independent_of_neighbours(ctx, tri_duct, prec):
     assert np.array_equal(z[2:], z0[::-1])
 
 
+def _tiny_meshes():
+    tri = (np.array([[-0.5, -0.4, 0.0], [0.6, -0.3, 0.1], [0.0, 0.7, -0.05]], np.float32), np.array([[0, 1, 2]]))
+    return [("one triangle", tri), ("plate 2 triangles", M.plate(1.0, 1.0, 1, 1)),
+            ("plate 4 triangles", M.plate(1.0, 1.0, 1, 2)), ("box 12 triangles", M.box(0.5))]
+
+
+@pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_FP64, S.SBR_MIXED])
+def test_tiny_meshes_root_leaf(ctx, prec):
+    """Meshes of one to twelve triangles, whose tree root is a leaf or whose
+    wide tree is a single record (nothing to stage in LDS): per ray against
+    the oracle, batched with either cull method against per-angle, and the
+    flat plate at normal incidence against sigma = 4 pi A^2 / lambda^2."""
+    for name, (v, f) in _tiny_meshes():
+        mesh = ctx.mesh(v, f)
+        sc = O.Scene(v, f, 0) if prec == S.SBR_FP32 else O.Scene(v, f, 1)
+        kw = dict(theta_start=0, theta_end=120, theta_samples=4, phi_start=10, phi_end=100, phi_samples=3,
+                  grid_size=2.0, n_x=64, n_y=48, max_bounces=4, freq=5e9)
+        p, op = S.SweepParams.make(**kw), O.Params(**kw)
+        for a in (0, 4, 7):
+            _step(f"precision {prec} {name} angle {a}: per ray")
+            g = ctx.trace_rays(mesh, p, a, prec)
+            o = sc.trace_angle(op, a, fused=prec == S.SBR_FP32)
+            assert np.array_equal(g["count"], o["count"])
+            assert np.array_equal(g["ids"], o["ids"])
+            assert np.array_equal(g["L"], o["L"])
+        jobs = S.sweep_jobs(p, mesh, prec)
+        pa = ctx.plan(jobs, prec, S.SBR_PER_ANGLE)
+        pa.run()
+        pa.wait()
+        z0, _, _ = pa.results()
+        pa.close()
+        assert np.count_nonzero(z0) > 0
+        pl = ctx.plan(jobs, prec, S.SBR_BATCHED)
+        for cull in (1, 2):
+            _step(f"precision {prec} {name}: batched cull={cull}")
+            pl.tune(cull=cull, top_levels=4)
+            pl.run()
+            pl.wait()
+            z, _, _ = pl.results()
+            assert np.array_equal(z, z0)
+        pl.close()
+        ctx.destroy_mesh(mesh)
+    # all rays hit the plate with the same path length: |sum| = A / lambda
+    v, f = M.plate(2.0, 2.0, 3, 5)
+    mesh = ctx.mesh(v, f)
+    g, n, freq = 1.2, 60, 3e9
+    p = S.SweepParams.make(theta_start=0, theta_end=0, theta_samples=1, phi_start=0, phi_end=0, phi_samples=1,
+                           grid_size=g, n_x=n, n_y=n, max_bounces=4, freq=freq)
+    _, sig, _ = ctx.sweep(mesh, p, prec, S.SBR_BATCHED)
+    lam = 299792458.0 / freq
+    rel = 1e-12 if prec != S.SBR_FP32 else 1e-5
+    assert sig[0] == pytest.approx(4 * np.pi * (g * g) ** 2 / lam ** 2, rel=rel)
+    ctx.destroy_mesh(mesh)
+
+
 @pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_FP64, S.SBR_MIXED])
 def test_non_square_ray_grid(ctx, tri_duct, prec):
     """n_x != n_y (the reference's two config keys, include/app/config.hpp:7-27):
