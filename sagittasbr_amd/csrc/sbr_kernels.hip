@@ -518,7 +518,14 @@ constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
 // along the ray by shift (first query): FP32 box distances are then shift
 // shorter than the FP64 ones. Unless amb is set, a hit's FP64 vertices and
 // face id are returned in hv0, hv1, hv2 and hFace.
-template <int SD, bool CNT, bool WW, bool TOP, int W>
+// OCT (four-wide tree only): every lane running the query has the same
+// direction signs. A slab's near and far planes are then known per axis for
+// the whole wavefront: the near plane's distance is the smaller of the two
+// (both are formed from the same rounded terms, and rounding is monotonic),
+// so the box test reads the near and far planes from their own offsets in the
+// record and needs no min / max per plane pair. Entries and decisions are
+// bit-identical to the general test.
+template <int SD, bool CNT, bool WW, bool TOP, int W, bool OCT = false>
 __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float> o, V3<float> inv, V3<double> o64,
                                                  V3<double> d64, double shift, double& tHit, int* sRef, float* sE,
                                                  Ov<float>::T* ov, const vf4* top, int nTop, int tid, Work& w,
@@ -567,31 +574,103 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     float noKey;
     asm("s_mov_b32 %0, 0x7f61b1e6" : "=s"(noKey));
     int cur = m.rootRefW[width_slot(W)];
+    // OCT: byte offsets of the near and far planes of each axis in a record
+    // (planes lo x, lo y, lo z, hi x, hi y, hi z of the four children, 16
+    // bytes each; a negative direction component enters at the hi plane), the
+    // wavefront's sign bits read from its first lane, and the grown origin
+    // terms of the near and far planes
+    static_assert(!OCT || W == 4, "the per-octant box test reads four-wide records");
+    const unsigned sgn = __builtin_amdgcn_readfirstlane((inv.x < 0.0f ? 1u : 0u) | (inv.y < 0.0f ? 2u : 0u) |
+                                                        (inv.z < 0.0f ? 4u : 0u));
+    const uint32_t nOx = (sgn & 1u) ? 48u : 0u, nOy = (sgn & 2u) ? 64u : 16u, nOz = (sgn & 4u) ? 80u : 32u;
+    const uint32_t fOx = 48u - nOx, fOy = 80u - nOy, fOz = 112u - nOz;
+    const V3<float> pN{(sgn & 1u) ? pHi.x : pLo.x, (sgn & 2u) ? pHi.y : pLo.y, (sgn & 4u) ? pHi.z : pLo.z};
+    const V3<float> pF{(sgn & 1u) ? pLo.x : pHi.x, (sgn & 2u) ? pLo.y : pHi.y, (sgn & 4u) ? pLo.z : pHi.z};
     // wide tree: the hit children are visited nearest first (the outcome
     // does not depend on the order)
     auto node_step = [&]() -> bool {
-        // records below nTop are the top levels the workgroup staged in LDS
-        const NodeW<W> n = (TOP && cur < nTop) ? load_nodew_lds<W>(top, cur) : load_nodew<W>(nodes, cur);
         if (CNT) w.visits += W;
-        float e[W];
-        int r[W];
+        if constexpr (OCT) {
+            vf4 nx, ny, nz, fx, fy, fz, rf;
+            if (TOP && cur < nTop) {
+                const char* b = reinterpret_cast<const char*>(top) + static_cast<uint32_t>(cur) * 128u;
+                nx = *reinterpret_cast<const vf4*>(b + nOx);
+                ny = *reinterpret_cast<const vf4*>(b + nOy);
+                nz = *reinterpret_cast<const vf4*>(b + nOz);
+                fx = *reinterpret_cast<const vf4*>(b + fOx);
+                fy = *reinterpret_cast<const vf4*>(b + fOy);
+                fz = *reinterpret_cast<const vf4*>(b + fOz);
+                rf = *reinterpret_cast<const vf4*>(b + 96);
+            } else {
+                // (each plane's scalar base plus the record's offset)
+                const uint32_t ob = static_cast<uint32_t>(cur) * 128u;
+                const char* nb = static_cast<const char*>(nodes);
+                nx = *at_byte<vf4>(nb + nOx, ob);
+                ny = *at_byte<vf4>(nb + nOy, ob);
+                nz = *at_byte<vf4>(nb + nOz, ob);
+                fx = *at_byte<vf4>(nb + fOx, ob);
+                fy = *at_byte<vf4>(nb + fOy, ob);
+                fz = *at_byte<vf4>(nb + fOz, ob);
+                rf = *at_byte<vf4>(nb + 96, ob);
+            }
+            auto key = [&](int q) {
+                const float tn = fmaxf(fmaxf(fmaxf(fmaf(nx[q], inv.x, -pN.x), fmaf(ny[q], inv.y, -pN.y)),
+                                             fmaf(nz[q], inv.z, -pN.z)),
+                                       0.001f);
+                const float tf = fminf(fminf(fmaf(fx[q], inv.x, -pF.x), fmaf(fy[q], inv.y, -pF.y)),
+                                       fmaf(fz[q], inv.z, -pF.z));
+                return (tn <= tf && tn <= lim) ? tn : noKey;
+            };
+            // the four (key, child) pairs as plain scalars: held in arrays,
+            // the compiler turns the sort's selects of children into selects
+            // of vector indices
+            float e0 = key(0), e1 = key(1), e2 = key(2), e3 = key(3);
+            int r0 = __float_as_int(rf.x), r1 = __float_as_int(rf.y), r2 = __float_as_int(rf.z),
+                r3 = __float_as_int(rf.w);
+            auto cx = [](float& ea, float& eb, int& ra, int& rb) {
+                const bool sw = eb < ea;
+                const float lo = sw ? eb : ea, hi = sw ? ea : eb;
+                const int rl = sw ? rb : ra, rh = sw ? ra : rb;
+                ea = lo;
+                eb = hi;
+                ra = rl;
+                rb = rh;
+            };
+            cx(e0, e1, r0, r1);
+            cx(e2, e3, r2, r3);
+            cx(e0, e2, r0, r2);
+            cx(e1, e3, r1, r3);
+            cx(e1, e2, r1, r2);
+            if (!(e0 < 1.0e38f)) return false;
+            if (e3 < 1.0e38f) st.push(e3, r3);
+            if (e2 < 1.0e38f) st.push(e2, r2);
+            if (e1 < 1.0e38f) st.push(e1, r1);
+            cur = r0;
+            return true;
+        } else {
+            float e[W];
+            int r[W];
+            // records below nTop are the top levels the workgroup staged in LDS
+            const NodeW<W> n = (TOP && cur < nTop) ? load_nodew_lds<W>(top, cur) : load_nodew<W>(nodes, cur);
 #pragma unroll
-        for (int q = 0; q < W; ++q) {
-            float ent;
-            // an unused slot's box lies at +infinity: its entry is +infinity
-            // (above the finite lim) or its slab interval is empty
-            const bool h = slab_grown(n.f[q], n.f[W + q], n.f[2 * W + q], n.f[3 * W + q], n.f[4 * W + q],
-                                      n.f[5 * W + q], pLo, pHi, inv, lim, ent);
-            e[q] = h ? ent : noKey;
-            r[q] = n.ref(q);
+            for (int q = 0; q < W; ++q) {
+                float ent;
+                // an unused slot's box lies at +infinity: its entry is
+                // +infinity (above the finite lim) or its slab interval is
+                // empty
+                const bool h = slab_grown(n.f[q], n.f[W + q], n.f[2 * W + q], n.f[3 * W + q], n.f[4 * W + q],
+                                          n.f[5 * W + q], pLo, pHi, inv, lim, ent);
+                e[q] = h ? ent : noKey;
+                r[q] = n.ref(q);
+            }
+            sort_net<W>(e, r);
+            if (!(e[0] < 1.0e38f)) return false;
+#pragma unroll
+            for (int q = W - 1; q >= 1; --q)
+                if (e[q] < 1.0e38f) st.push(e[q], r[q]);
+            cur = r[0];
+            return true;
         }
-        sort_net<W>(e, r);
-        if (!(e[0] < 1.0e38f)) return false;
-#pragma unroll
-        for (int q = W - 1; q >= 1; --q)
-            if (e[q] < 1.0e38f) st.push(e[q], r[q]);
-        cur = r[0];
-        return true;
     };
     auto leaf_tests = [&]() {
         int code = ~cur;
@@ -936,6 +1015,11 @@ __device__ __forceinline__ void ray_init_mixed(const Job<double>& D, int i, int 
     st.k = 0;
 }
 
+#ifndef SBR_OCTANT_TEST
+#define SBR_OCTANT_TEST 1
+#endif
+constexpr bool kOctantTest = SBR_OCTANT_TEST != 0;
+
 // One query of bounce st.k; returns true when the ray has ended (miss or
 // max_bounces reached). A query whose answer the mixed query cannot vouch for
 // (a near-tie or a hit on the edge of its box) is answered by the reference
@@ -968,8 +1052,16 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
     bool amb = false;
     V3<double> v0, v1, v2, e1, e2;
     int face;
-    int h = closest_hit_mixed<SD, CNT, WW, TOP, W>(M, of, inv, o, d, shift, t, sRef, sE, ov, top, nTop, tid, w, ex, ed,
-                                                K, amb, v0, v1, v2, face);
+    int h;
+    // the same direction signs in every lane running this query (always so
+    // in a first query: the job's rays are parallel): the per-octant box test
+    const unsigned sg = (inv.x < 0.0f ? 1u : 0u) | (inv.y < 0.0f ? 2u : 0u) | (inv.z < 0.0f ? 4u : 0u);
+    if (W == 4 && kOctantTest && __builtin_amdgcn_ballot_w64(sg != __builtin_amdgcn_readfirstlane(sg)) == 0)
+        h = closest_hit_mixed<SD, CNT, WW, TOP, W, W == 4>(M, of, inv, o, d, shift, t, sRef, sE, ov, top, nTop, tid,
+                                                            w, ex, ed, K, amb, v0, v1, v2, face);
+    else
+        h = closest_hit_mixed<SD, CNT, WW, TOP, W>(M, of, inv, o, d, shift, t, sRef, sE, ov, top, nTop, tid, w, ex,
+                                                    ed, K, amb, v0, v1, v2, face);
     if (CNT) queries++;
     // FP64 edges of the chosen triangle (rtKernels.cu:13-14)
     if (!amb) {
