@@ -1031,7 +1031,7 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
                                                Ov<double>::T* ov64, const vf4* top, int nTop, int tid, int32_t* ids,
                                                Work& w,
                                                unsigned long long& queries, unsigned long long& fullQ, float K,
-                                               bool& fell) {
+                                               bool& fell, double* sCos) {
     const V3<double> o = st.o, d = st.d;
     // FP32 copy of the ray for the box tests; the launch-plane move only
     // shifts the origin along the ray, which changes no decision
@@ -1085,7 +1085,9 @@ __device__ __forceinline__ bool ray_step_mixed(const Job<float>& J, const Mesh<f
     // same vertices)
     V3<double> n = unit3(cross3(e1, e2));
     V3<double> hp = axpy(t, d, o);
-    st.P.cosT = dot3(n, V3<double>{-d.x, -d.y, -d.z});
+    // (the lane's LDS slot: only the last bounce's value is used, after the
+    // path ends, and as a register it was spilled to scratch every bounce)
+    sCos[tid] = dot3(n, V3<double>{-d.x, -d.y, -d.z});
     st.P.count++;
     st.P.L += t;
     if (REC) ids[st.k] = face;
@@ -1105,13 +1107,14 @@ __device__ __forceinline__ PathOut<double> trace_path_mixed(const Job<float>& J,
                                                             Ov<double>::T* ov64, const vf4* top, int nTop, int tid,
                                                             int32_t* ids, Work& w,
                                                             unsigned long long& queries, unsigned long long& fullQ,
-                                                            float K, bool& fell) {
+                                                            float K, bool& fell, double* sCos) {
     RayState<double> st;
     ray_init_mixed(D, i, j, st);
     if (J.maxB > 0)
         while (!ray_step_mixed<SD, CNT, REC, WW, TOP, W>(J, M, M64, st, sRef, sE, ov, ov64, top, nTop, tid, ids, w,
-                                                      queries, fullQ, K, fell)) {
+                                                      queries, fullQ, K, fell, sCos)) {
         }
+    if (st.P.count > 0) st.P.cosT = sCos[tid];
     st.P.Ld = st.P.L;
     return st.P;
 }
@@ -1308,6 +1311,8 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
     __shared__ CullRect sCull[REC ? 1 : kCullBoxes];
     // the top levels of the current mesh's W-wide tree (TOP)
     __shared__ vf4 sTop[TOP ? kTopBytes / 16 : 1];
+    // each lane's last-bounce incidence cosine (mixed mode)
+    __shared__ double sCos[MODE == 1 ? kWgSize : 1];
     int stagedMesh = -1, nTop = 0;
     static_assert(kCullBoxes == kWgSize, "one cull box per thread");
 
@@ -1452,7 +1457,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                         if constexpr (MODE == 1) {
                             done = ray_step_mixed<SD, CNT, false, WW, false, kWide>(
                                 J, M, a.meshes64[J.mesh], st, sRef, sE, ov, ov64, nullptr, 0, tid, nullptr, w,
-                                cQueries, cFull, a.errK, fell);
+                                cQueries, cFull, a.errK, fell, sCos);
                         } else {
                             done = ray_step<R, SD, CNT, false, WW, false>(J, M, st, sRef, sE, ov, tid, nullptr, w,
                                                                    cQueries);
@@ -1460,6 +1465,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                         if (done) {
                             ST re = ST(0), im = ST(0);
                             if constexpr (MODE == 1) {
+                                if (st.P.count > 0) st.P.cosT = sCos[tid];
                                 st.P.Ld = st.P.L;
                                 po_term<double, true>(a.jobs64[jid], st.P, re, im);
                                 if (CNT && fell) cUndecided++;
@@ -1550,7 +1556,7 @@ __global__ __launch_bounds__(kWgSize) __attribute__((amdgpu_waves_per_eu(kWavesP
                         const Job<double>& D = a.jobs64[jid];
                         const PathOut<double> PD = trace_path_mixed<SD, CNT, REC, WW, TOP, W>(
                             J, D, M, a.meshes64[J.mesh], i, j, sRef, sE, ov, ov64, sTop, nTop, tid, ids, w, cQueries,
-                            cFull, a.errK, fell);
+                            cFull, a.errK, fell, sCos);
                         po_term<double, true>(D, PD, dre, dimg);
                         P.count = PD.count;
                         P.Ld = PD.Ld;
