@@ -127,6 +127,29 @@ def test_per_ray_mixed_has_fp64_outcome(ctx, sphere, tri_duct, case, angles):
         assert flagged > 0, "the FP64 answer path was not exercised"
 
 
+@pytest.mark.parametrize("case", ["sphere", "tri_duct"])
+def test_per_ray_mixed_every_octant(ctx, sphere, tri_duct, case):
+    """Launch directions in all eight octants and along the axes (exactly
+    zero direction components, signed zeros included): the mixed query takes
+    its per-octant box test whenever a wavefront's rays share their direction
+    signs (every first query) and the general test otherwise; either way every
+    ray ends with the FP64 oracle's outcome."""
+    v, f, mesh, s32, s64 = sphere if case == "sphere" else tri_duct
+    dirs = [(t, ph) for t in (30, 150) for ph in (45, 135, 225, 315)] + [(0, 0), (90, 0), (90, 90), (90, 180),
+                                                                          (90, 270), (180, 0)]
+    for th, ph in dirs:
+        kw = dict(theta_start=th, theta_end=th, theta_samples=1, phi_start=ph, phi_end=ph, phi_samples=1,
+                  grid_size=2.5 if case == "sphere" else 1.6, n_x=160, n_y=160, max_bounces=6, freq=10e9)
+        p, op = S.SweepParams.make(**kw), O.Params(**kw)
+        _step(f"{case} theta {th} phi {ph}")
+        g = ctx.trace_rays(mesh, p, 0, S.SBR_MIXED)
+        o = s64.trace_angle(op, 0)
+        assert np.array_equal(g["count"], o["count"])
+        assert np.array_equal(g["ids"], o["ids"])
+        assert np.array_equal(g["L"], o["L"])
+        assert np.count_nonzero(g["count"]) > 0
+
+
 @pytest.mark.parametrize("scale", [0.37, 2.0])
 def test_per_ray_mixed_scaled_model(ctx, scale):
     """A model scale that rounds the FP32 vertices (0.37) makes the mixed
