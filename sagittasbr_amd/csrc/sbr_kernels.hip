@@ -579,7 +579,8 @@ __device__ __forceinline__ int closest_hit_mixed(const Mesh<float>& m, V3<float>
     // bytes each; a negative direction component enters at the hi plane), the
     // wavefront's sign bits read from its first lane, and the grown origin
     // terms of the near and far planes
-    static_assert(!OCT || W == 4, "the per-octant box test reads four-wide records");
+    static_assert(!OCT || (W == 4 && kRecBytes<W> == 128 && NodeW<W>::kLoads == 7),
+                  "the per-octant box test reads 128-byte four-wide records: six planes, then the references");
     const unsigned sgn = __builtin_amdgcn_readfirstlane((inv.x < 0.0f ? 1u : 0u) | (inv.y < 0.0f ? 2u : 0u) |
                                                         (inv.z < 0.0f ? 4u : 0u));
     const uint32_t nOx = (sgn & 1u) ? 48u : 0u, nOy = (sgn & 2u) ? 64u : 16u, nOz = (sgn & 4u) ? 80u : 32u;
