@@ -417,6 +417,40 @@ def test_tri_duct_sigma_fp64_and_methods(ctx, tri_duct):
     np.testing.assert_allclose(z32, r32, rtol=2e-4, atol=2e-4 * np.abs(r32).max())
 
 
+@pytest.mark.parametrize("refl", [0.5, 1.25])
+def test_reflection_constant(ctx, tri_duct, refl):
+    """The reflection constant R enters each ray's PO term as R^n over its n
+    bounces (poKernels.cu:83-86): sigma of every precision against the
+    oracle at the same R, and per ray, R scales the mixed mode's R = 1
+    contribution by R^n (a property independent of the oracle)."""
+    v, f, mesh, s32, s64 = tri_duct
+    kw = dict(C2S, theta_samples=3, phi_samples=3, reflection_const=refl)
+    p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    _step(f"R {refl}: oracle fp64 sweep")
+    ref, _ = s64.sweep(op)
+    for prec in (S.SBR_FP64, S.SBR_MIXED):
+        _step(f"R {refl}: precision {prec} batched")
+        z, _, _ = ctx.sweep(mesh, p, prec, S.SBR_BATCHED)
+        np.testing.assert_allclose(z, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+    _step(f"R {refl}: oracle fp32 sweep and fp32 batched")
+    r32, _ = s32.sweep(op, fused=True)
+    z32, _, _ = ctx.sweep(mesh, p, S.SBR_FP32, S.SBR_BATCHED)
+    np.testing.assert_allclose(z32, r32, rtol=2e-4, atol=2e-4 * np.abs(r32).max())
+    p1 = S.SweepParams.make(**dict(kw, reflection_const=1.0))
+    multi = 0
+    for a in (0, 4, 8):
+        _step(f"R {refl}: per ray, angle {a}")
+        g = ctx.trace_rays(mesh, p, a, S.SBR_MIXED)
+        g1 = ctx.trace_rays(mesh, p1, a, S.SBR_MIXED)
+        assert np.array_equal(g["count"], g1["count"])
+        scale = float(refl) ** g["count"].astype(np.float64)
+        np.testing.assert_allclose(g["reim"], g1["reim"] * scale[:, None], rtol=1e-13,
+                                   atol=1e-15 * np.abs(g1["reim"]).max())
+        multi += np.count_nonzero(g["count"] > 1)
+    assert multi > 0, "no multi-bounce ray exercised R^n"
+
+
+
 def test_counters_match_oracle(ctx, tri_duct):
     v, f, mesh, s32, s64 = tri_duct
     kw = dict(C2S, theta_samples=1, theta_start=45, theta_end=45, phi_samples=1, phi_start=45, phi_end=45)
