@@ -417,6 +417,50 @@ def test_tri_duct_sigma_fp64_and_methods(ctx, tri_duct):
     np.testing.assert_allclose(z32, r32, rtol=2e-4, atol=2e-4 * np.abs(r32).max())
 
 
+def _corridor():
+    """Two facing plates 0.2 m apart, 8 m deep below z = 0: rays entering at
+    theta 28-35 degrees bounce between them 20 times and more."""
+    va, fa = M._grid_quad([-0.1, -0.5, -8.0], [0, 1, 0], [0, 0, 8], 4, 32)
+    vb, fb = M._grid_quad([0.1, -0.5, -8.0], [0, 0, 8], [0, 1, 0], 32, 4)
+    return np.concatenate([va, vb]).astype(np.float32), np.concatenate([fa, fb + len(va)]).astype(np.uint32)
+
+
+@pytest.mark.parametrize("prec", [S.SBR_FP32, S.SBR_FP64, S.SBR_MIXED])
+def test_bounce_limits(ctx, prec):
+    """max_bounces 1 (every hit ends its ray) and 20 (the reference's
+    default, include/app/config.hpp:8-25), on two facing plates where rays
+    reach 20 bounces: per ray against the oracle, and the batched sums
+    against per-angle."""
+    v, f = _corridor()
+    mesh = ctx.mesh(v, f)
+    s32, s64 = O.Scene(v, f, 0), O.Scene(v, f, 1)
+    for mb in (1, 20):
+        kw = dict(theta_start=28, theta_end=35, theta_samples=3, phi_start=0, phi_end=5, phi_samples=2,
+                  grid_size=2.0, n_x=400, n_y=400, max_bounces=mb, freq=10e9)
+        p, op = S.SweepParams.make(**kw), O.Params(**kw)
+        top = 0
+        for a in (0, 3, 5):
+            _step(f"precision {prec} max_bounces {mb} angle {a}")
+            g = ctx.trace_rays(mesh, p, a, prec)
+            o = s32.trace_angle(op, a, fused=True) if prec == S.SBR_FP32 else s64.trace_angle(op, a)
+            assert np.array_equal(g["count"], o["count"])
+            assert np.array_equal(g["ids"], o["ids"])
+            assert np.array_equal(g["L"], o["L"])
+            assert g["count"].max() <= mb
+            top += np.count_nonzero(g["count"] == mb)
+        assert top > 0, "no ray reached the bounce limit"
+        jobs = S.sweep_jobs(p, mesh, prec)
+        sums = []
+        for method in (S.SBR_PER_ANGLE, S.SBR_BATCHED):
+            pl = ctx.plan(jobs, prec, method)
+            pl.run()
+            pl.wait()
+            sums.append(pl.results()[0])
+            pl.close()
+        assert np.array_equal(sums[0], sums[1])
+    ctx.destroy_mesh(mesh)
+
+
 @pytest.mark.parametrize("refl", [0.5, 1.25])
 def test_reflection_constant(ctx, tri_duct, refl):
     """The reflection constant R enters each ray's PO term as R^n over its n
