@@ -417,6 +417,46 @@ def test_tri_duct_sigma_fp64_and_methods(ctx, tri_duct):
     np.testing.assert_allclose(z32, r32, rtol=2e-4, atol=2e-4 * np.abs(r32).max())
 
 
+def test_duplicate_and_degenerate_triangles(ctx):
+    """A sphere with a quarter of its faces repeated (every ray that hits one
+    of them meets two triangles at exactly the same distance: the reference
+    keeps the one its tree order tests last, rtKernels.cu:34-40) and three
+    degenerate faces (a repeated vertex, twice, and three collinear vertices,
+    rejected by the determinant test): per ray against the oracle in every
+    precision. The mixed mode answers the exact ties with the FP64 query."""
+    v, f = M.icosphere(3)
+    v, f = np.asarray(v, np.float32), np.asarray(f, np.uint32)
+    rng = np.random.default_rng(5)
+    dup = f[rng.choice(len(f), len(f) // 4, replace=False)]
+    nv = len(v)
+    line = np.array([[0.2, 0.1, 1.5], [0.4, 0.3, 1.5], [0.6, 0.5, 1.5]], np.float32)
+    degen = np.array([[0, 0, 1], [5, 7, 5], [nv, nv + 1, nv + 2]], np.uint32)
+    V, F = np.concatenate([v, line]), np.concatenate([f, dup, degen]).astype(np.uint32)
+    mesh = ctx.mesh(V, F)
+    s32, s64 = O.Scene(V, F, 0), O.Scene(V, F, 1)
+    kw = dict(theta_start=0, theta_end=90, theta_samples=2, phi_start=0, phi_end=90, phi_samples=2, grid_size=2.5,
+              n_x=200, n_y=200, max_bounces=5, freq=3e9)
+    p, op = S.SweepParams.make(**kw), O.Params(**kw)
+    flagged = dup_wins = 0
+    for a in range(4):
+        o64 = s64.trace_angle(op, a)
+        o32 = s32.trace_angle(op, a, fused=True)
+        for prec in (S.SBR_FP32, S.SBR_FP64, S.SBR_MIXED):
+            _step(f"angle {a} precision {prec}")
+            g = ctx.trace_rays(mesh, p, a, prec)
+            o = o32 if prec == S.SBR_FP32 else o64
+            assert np.array_equal(g["count"], o["count"])
+            assert np.array_equal(g["ids"], o["ids"])
+            assert np.array_equal(g["L"], o["L"])
+            if prec == S.SBR_MIXED:
+                flagged += int(np.count_nonzero(g["retraced"]))
+        # hits won by a repeated face (ids past the original faces)
+        dup_wins += int(np.count_nonzero((o64["ids"] >= len(f)) & (o64["ids"] < len(f) + len(dup))))
+    assert dup_wins > 0
+    assert flagged > 0, "no exact tie reached the FP64 query"
+    ctx.destroy_mesh(mesh)
+
+
 def _corridor():
     """Two facing plates 0.2 m apart, 8 m deep below z = 0: rays entering at
     theta 28-35 degrees bounce between them 20 times and more."""
